@@ -1,0 +1,177 @@
+/*
+ * rsvdt_hip.h -- thin C ABI over the hand-written gfx950 HIP kernels.
+ *
+ * Plain pointers, sizes and an opaque stream; no torch or PETSc types.
+ * Every function returns 0 on success or a nonzero RSVDT_HIP_* code, and
+ * never allocates device memory unless its name says so.  All pointers are
+ * device pointers unless the parameter name ends in _host.
+ *
+ * Storage conventions
+ *   complex values are interleaved (re, im) pairs of double (z) or float (c);
+ *   a "block-vector array" holds nb blocks of an N x k row-major block, block
+ *   b at offset b*N*k: entry (b, row, j) at [b*N*k + row*k + j].  The sketch
+ *   F_hat / response Y_hat is such an array with one block per frequency, the
+ *   snapshot store with one block per snapshot.
+ *
+ * These functions sit below the reference-compatible entry points declared
+ * in rsvdt.h; the reference has no counterpart for them (it calls PETSc).
+ */
+#ifndef RSVDT_HIP_H
+#define RSVDT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RSVDT_HIP_OK = 0,
+    RSVDT_HIP_ERR_RUNTIME = 101,     /* a HIP runtime call failed */
+    RSVDT_HIP_ERR_ARG = 102,         /* unsupported size / bad argument */
+    RSVDT_HIP_ERR_NODEV = 103,       /* no GPU visible */
+    RSVDT_HIP_ERR_NUMERIC = 104      /* breakdown (non-positive pivot, singular solve) */
+};
+
+typedef void *rsvdt_stream_t;        /* hipStream_t, or NULL for the null stream */
+
+/* ---- runtime ---------------------------------------------------------- */
+int rsvdt_hip_init(int device);                       /* select device, check gfx950 */
+const char *rsvdt_hip_last_error(void);               /* text of the last failure */
+int rsvdt_hip_malloc(void **p, size_t bytes);
+int rsvdt_hip_free(void *p);
+int rsvdt_hip_memcpy_h2d(void *dst, const void *src_host, size_t bytes);
+int rsvdt_hip_memcpy_d2h(void *dst_host, const void *src, size_t bytes);
+int rsvdt_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, rsvdt_stream_t s);
+int rsvdt_hip_zero(void *p, size_t bytes, rsvdt_stream_t s);  /* fill kernel, graph-safe */
+int rsvdt_hip_sync(void);
+int rsvdt_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* events for timing on a given stream */
+int rsvdt_hip_event_create(void **ev);
+int rsvdt_hip_event_destroy(void *ev);
+int rsvdt_hip_event_record(void *ev, rsvdt_stream_t s);
+int rsvdt_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- fused CSR SpMM + RK4 stage (replaces TSRK4.c:18-47) --------------- *
+ * One of the four classical RK4 stages on nb independent N x k blocks,
+ * sharing the operator (CSR, int32 indices):
+ *   s      = A * uin + F                (F may be NULL: homogeneous)
+ *   stage 1: uout = y + c_next*s ; acc  = y   + c_acc*s     (uin == y)
+ *   stage 2,3: uout = y + c_next*s; acc = acc + c_acc*s
+ *   stage 4: y    = acc + c_acc*s ; snap (if non-NULL) = y (FP64 copy)
+ * prec: 64 = complex FP64 state, 32 = complex FP32 state.
+ * a_real: values are real (length nnz) rather than complex (2*nnz).
+ * Value arrays must be of the state precision.  Block strides are N*k.
+ * lanes: lanes cooperating on one row (power of two, 0 = automatic).  */
+typedef struct {
+    int64_t nrows;
+    int64_t k;            /* columns per block */
+    int64_t nb;           /* number of blocks stepped together */
+    const int32_t *rowptr;
+    const int32_t *col;
+    const void *val;
+    int a_real;
+    int prec;
+    int lanes;            /* 0 = default choice */
+    int xcd_remap;        /* 1 = give each XCD a contiguous row range */
+} rsvdt_spmm_desc;
+
+int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,
+                        const void *uin, const void *y_or_null_stage4, const void *F,
+                        void *acc, void *out, double c_next, double c_acc,
+                        double *snap, rsvdt_stream_t s);
+
+/* Plain block SpMM: out_b = M * in_b for nb blocks of k columns (complex FP64
+ * values, int32 CSR).  in blocks are ncols x k (stride in_bstride), out blocks
+ * nrows x k (stride out_bstride).  Used for the B, C and weight matrices.   */
+int rsvdt_hip_spmm_blocks(int64_t nrows, int64_t k, int64_t nb, const int32_t *rowptr, const int32_t *col,
+                          const double *val_complex, const double *in, int64_t in_bstride,
+                          double *out, int64_t out_bstride, rsvdt_stream_t s);
+
+/* ---- small-coefficient complex GEMM (forcing chunks, DFT, windows) ----- *
+ * out[p][l] = alpha * sum_{q<K} coef(q,p) * in[q*ldi + l]   (+ beta*out)
+ * for p < P, l < L (complex entries), with the coefficient column map
+ *   coef(q,p) = coef[q*cq + ((p0 + sgn*p) mod cmod)*cp]
+ * (sgn = +1 or -1), which lets the forcing read iDFT samples straight from
+ * the 2Ns-periodic table in either time direction.  coef and in are complex
+ * FP64; out is FP64 (out_prec 64) or FP32 (out_prec 32).  Used for the
+ * forcing chunk (in = F_hat frequency blocks, coef = iDFT table), the DFT
+ * (in = snapshots, coef = RR*dft) and the transient-removal windows.      */
+int rsvdt_hip_coef_gemm(int64_t P, int64_t K, int64_t L,
+                        const double *coef, int64_t cq, int64_t cp,
+                        int64_t p0, int sgn, int64_t cmod, double alpha,
+                        const double *in, int64_t ldi,
+                        void *out, int64_t ldo, int out_prec,
+                        double beta, rsvdt_stream_t s);
+
+/* ---- batched tall-skinny inner products (Gram / projections) ---------- *
+ * G_b[a][c] = sum_{r<nrows} conj(X_b(r,a)) * Z_b(r,c), a < na, c < nc,
+ * X_b(r,a) = X[b*xb + r*xr + a*xa] (complex FP64), same for Z.
+ * G is nbatch x na x nc complex FP64 (row-major per batch).  Deterministic
+ * two-pass reduction; work must hold rsvdt_hip_bgram_work_bytes(...) bytes. */
+size_t rsvdt_hip_bgram_work_bytes(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc);
+int rsvdt_hip_bgram(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc,
+                    const double *X, int64_t xb, int64_t xr, int64_t xa,
+                    const double *Z, int64_t zb, int64_t zr, int64_t zc,
+                    double *G, void *work, rsvdt_stream_t s);
+
+/* ---- batched row-local small products --------------------------------- *
+ * Out_b(r,c) = alpha * Y_b(r,c) + sum_{a<na} X_b(r,a) * C_b[a][c]
+ * (Y may be NULL).  C is nbatch x na x nc complex FP64 row-major.
+ * Out may alias X (in-place X <- X*C) when the strides agree.            */
+int rsvdt_hip_bapply(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc,
+                     const double *X, int64_t xb, int64_t xr, int64_t xa,
+                     const double *C,
+                     double alpha, const double *Y, int64_t yb, int64_t yr, int64_t yc,
+                     double *Out, int64_t ob, int64_t orow, int64_t oc,
+                     rsvdt_stream_t s);
+
+/* ---- batched small dense kernels (one workgroup per matrix) ----------- */
+/* Cholesky of (G_b + shift_b*I) (upper R, G = R^H R) then Rinv_b = R^-1.
+ * G, R, Rinv: nbatch x n x n complex FP64 row-major.  shift_rel >= 0:
+ * shift_b = shift_rel * trace(G_b).  info (device int, may be NULL)
+ * receives the count of non-positive pivots (clamped).                  */
+int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel,
+                        double *R, double *Rinv, int *info, rsvdt_stream_t s);
+/* Rout_b = Ra_b * Rb_b for upper-triangular n x n (row-major). */
+int rsvdt_hip_btrmm_upper(int64_t nbatch, int64_t n, const double *Ra, const double *Rb,
+                          double *Rout, rsvdt_stream_t s);
+/* One-sided Jacobi SVD of R_b (n x n): R = U diag(sig) W^H, sig descending.
+ * U: nbatch x n x n row-major; sig: nbatch x n.  Each column of U is scaled
+ * so that the sum of its entries is real and positive.  n <= 64.          */
+int rsvdt_hip_bjacobi_svd(int64_t nbatch, int64_t n, const double *R, double *U,
+                          double *sig, rsvdt_stream_t s);
+/* Solve (shift_{b,m} I - M_b) X = B for every (b, m): M is nbatch x n x n,
+ * B and X are nbatch x n x nrhs (row-major, X may alias B), shift is
+ * nbatch x nrhs complex.  LU with partial pivoting; work must hold
+ * rsvdt_hip_bshift_solve_work_bytes(...) bytes.                           */
+size_t rsvdt_hip_bshift_solve_work_bytes(int64_t nbatch, int64_t n, int64_t nrhs);
+int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs,
+                           const double *M, const double *shift,
+                           const double *B, double *X, void *work, int *info,
+                           rsvdt_stream_t s);
+
+/* ---- elementwise ------------------------------------------------------ */
+/* y = a*x (complex entries, FP64), n entries */
+int rsvdt_hip_zscale(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s);
+/* precision conversions of complex arrays */
+int rsvdt_hip_z2c(int64_t n, const double *x, float *y, rsvdt_stream_t s);
+int rsvdt_hip_c2z(int64_t n, const float *x, double *y, rsvdt_stream_t s);
+/* strided complex copy: dst[b*db + r*dr + c*dc] = src[b*sb + r*sr + c*sc] */
+int rsvdt_hip_zcopy3(int64_t nb, int64_t nr, int64_t nc,
+                     const double *src, int64_t sb, int64_t sr, int64_t sc,
+                     double *dst, int64_t db, int64_t dr, int64_t dc, rsvdt_stream_t s);
+/* Seeded sketch: a rows x (kb*nblk) complex matrix in kb-column row-major
+ * blocks; entry (r, c) is drawn from the counter-based stream at index
+ * c*rows + r (uniform [0,1) + i[0,1) when gauss = 0, complex N(0,1) when 1). */
+int rsvdt_hip_fill_random(int64_t rows, int64_t kb, int64_t nblk, uint64_t seed, int gauss, double *out,
+                          rsvdt_stream_t s);
+/* sum of |x|^2 over n complex entries into *out (device double) */
+int rsvdt_hip_znorm2(int64_t n, const double *x, double *out_dev, void *work, rsvdt_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

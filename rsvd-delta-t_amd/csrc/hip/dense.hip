@@ -1,0 +1,348 @@
+// Batched small dense linear algebra for gfx950, one workgroup per matrix:
+//   * Cholesky (with an optional diagonal shift) and triangular inverse: the
+//     small factorizations of (shifted) CholeskyQR, which replaces SLEPc's
+//     BVOrthogonalize in SourceCode/QRAllFreqs.c:22-32 and QRDecomposition.c:18;
+//   * one-sided Jacobi SVD of the k x k triangular factor, replacing SLEPc's
+//     SVDSolve in SourceCode/SVDAllFreqsBeforeAdjoint.c:31-49 and SVDAllFreqs.c:34-56;
+//   * LU solves of (e^{+-i w m dT} I - M_tilde) a = b for every (test vector,
+//     frequency), replacing the PGMRES KSPSolve of SourceCode/GalerkinProj.c:57-62
+//     by the exact solve it converges to.
+// The matrices are tiny (n <= 130) and there are tens to a few thousand of
+// them, so these kernels are latency-bound; one launch covers all of them.
+#include "common.h"
+
+namespace rsvdt {
+
+// ------------------------------------------------------------ Cholesky + inverse
+__global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restrict__ G, double shift_rel,
+                                                   double2 *__restrict__ R, double2 *__restrict__ Rinv, int *info)
+{
+    const int64_t b = blockIdx.x;
+    const double2 *Gb = G + b * n * n;
+    double2 *Rb = R + b * n * n;
+    double2 *Xb = Rinv + b * n * n;
+    __shared__ double s_shift;
+    __shared__ double s_piv;
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) {
+        double tr = 0;
+        for (int i = 0; i < n; ++i) tr += Gb[i * n + i].x;
+        s_shift = shift_rel * tr;
+        s_bad = 0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        double2 v = (i <= c) ? Gb[e] : make_double2(0, 0);
+        if (i == c) { v.x += s_shift; v.y = 0; }
+        Rb[e] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (threadIdx.x == 0) {
+            double d = Rb[j * n + j].x;
+            const double tiny = 1e-300;
+            if (!(d > tiny)) { d = tiny; s_bad++; }
+            s_piv = sqrt(d);
+            Rb[j * n + j] = make_double2(s_piv, 0);
+        }
+        __syncthreads();
+        const double inv = 1.0 / s_piv;
+        for (int c = j + 1 + threadIdx.x; c < n; c += blockDim.x) {
+            double2 v = Rb[j * n + c];
+            Rb[j * n + c] = make_double2(v.x * inv, v.y * inv);
+        }
+        __syncthreads();
+        const int m = n - j - 1;
+        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+            const int i = j + 1 + e / m, c = j + 1 + e % m;
+            if (i <= c) {
+                const double2 ri = Rb[j * n + i], rc = Rb[j * n + c];
+                double2 v = Rb[i * n + c];
+                // v -= conj(ri) * rc
+                v.x -= ri.x * rc.x + ri.y * rc.y;
+                v.y -= ri.x * rc.y - ri.y * rc.x;
+                Rb[i * n + c] = v;
+            }
+        }
+        __syncthreads();
+    }
+    // inverse of the upper-triangular factor, one column per thread
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        for (int i = c + 1; i < n; ++i) Xb[i * n + c] = make_double2(0, 0);
+        const double rcc = Rb[c * n + c].x;
+        Xb[c * n + c] = make_double2(1.0 / rcc, 0);
+        for (int i = c - 1; i >= 0; --i) {
+            double2 s = make_double2(0, 0);
+            for (int l = i + 1; l <= c; ++l) s = zadd(s, zmul(Rb[i * n + l], Xb[l * n + c]));
+            const double rii = Rb[i * n + i].x;
+            Xb[i * n + c] = make_double2(-s.x / rii, -s.y / rii);
+        }
+    }
+    if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
+}
+
+__global__ void k_btrmm_upper(int n, const double2 *__restrict__ A, const double2 *__restrict__ B,
+                              double2 *__restrict__ C)
+{
+    const int64_t b = blockIdx.x;
+    const double2 *Ab = A + b * n * n, *Bb = B + b * n * n;
+    double2 *Cb = C + b * n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        double2 s = make_double2(0, 0);
+        for (int l = i; l <= c; ++l) s = zadd(s, zmul(Ab[i * n + l], Bb[l * n + c]));
+        Cb[e] = s;
+    }
+}
+
+// ------------------------------------------------------------ Jacobi SVD
+// Columns of R live in LDS (column-major); a round-robin tournament pairs the
+// columns so that n/2 rotations run concurrently, one thread per pair.
+#define JAC_MAXN 64
+__global__ __launch_bounds__(256) void k_bjacobi_svd(int n, const double2 *__restrict__ R, double2 *__restrict__ U,
+                                                     double *__restrict__ sig)
+{
+    __shared__ double2 A[JAC_MAXN * JAC_MAXN];
+    __shared__ double nrm[JAC_MAXN];
+    __shared__ int ord[JAC_MAXN];
+    __shared__ int s_rot;
+    const int64_t b = blockIdx.x;
+    const double2 *Rb = R + b * n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        A[c * n + i] = Rb[e];
+    }
+    __syncthreads();
+    const int ne = (n + 1) & ~1;       // players, padded to even
+    const int m = ne - 1;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        if (threadIdx.x == 0) s_rot = 0;
+        __syncthreads();
+        for (int rnd = 0; rnd < m; ++rnd) {
+            const int p = threadIdx.x;
+            if (p < ne / 2) {
+                int i, j;
+                if (p == 0) { i = rnd; j = m; }
+                else { i = (rnd + p) % m; j = (rnd - p + m) % m; }
+                if (i > j) { int t = i; i = j; j = t; }
+                if (j < n) {
+                    double2 *ci = A + i * n, *cj = A + j * n;
+                    double al = 0, be = 0;
+                    double2 ga = make_double2(0, 0);
+                    for (int r = 0; r < n; ++r) {
+                        const double2 x = ci[r], y = cj[r];
+                        al += zabs2(x);
+                        be += zabs2(y);
+                        ga.x += x.x * y.x + x.y * y.y;
+                        ga.y += x.x * y.y - x.y * y.x;
+                    }
+                    const double ag = sqrt(zabs2(ga));
+                    if (ag > 0 && ag > 1e-15 * sqrt(al * be)) {
+                        s_rot = 1;
+                        const double2 eph = make_double2(ga.x / ag, -ga.y / ag);   // e^{-i phi}
+                        const double zeta = (be - al) / (2.0 * ag);
+                        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                        for (int r = 0; r < n; ++r) {
+                            const double2 xp = ci[r];
+                            const double2 xq = zmul(cj[r], eph);
+                            ci[r] = make_double2(cs * xp.x - sn * xq.x, cs * xp.y - sn * xq.y);
+                            cj[r] = make_double2(sn * xp.x + cs * xq.x, sn * xp.y + cs * xq.y);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (!s_rot) break;
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        double s = 0;
+        for (int r = 0; r < n; ++r) s += zabs2(A[c * n + r]);
+        nrm[c] = sqrt(s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < n; ++c) ord[c] = c;
+        for (int i = 1; i < n; ++i) {          // insertion sort, descending, stable
+            int t = ord[i], j = i - 1;
+            while (j >= 0 && nrm[ord[j]] < nrm[t]) { ord[j + 1] = ord[j]; --j; }
+            ord[j + 1] = t;
+        }
+    }
+    __syncthreads();
+    // phase convention: sum_i U(i, j) real and positive (largest-modulus entry
+    // when the sum vanishes), so the vectors do not depend on the rotation order
+    __shared__ double2 phs[JAC_MAXN];
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const double s = nrm[c];
+        double2 sum = make_double2(0, 0), bigv = make_double2(1, 0);
+        double big = -1;
+        for (int r = 0; r < n; ++r) {
+            const double2 v = A[c * n + r];
+            const double2 u = s > 0 ? make_double2(v.x / s, v.y / s) : make_double2(0, 0);
+            sum = zadd(sum, u);
+            const double m2 = zabs2(u);
+            if (m2 > big) { big = m2; bigv = u; }
+        }
+        const double as = sqrt(zabs2(sum)), ab = sqrt(zabs2(bigv));
+        phs[c] = as > 1e-8 ? make_double2(sum.x / as, -sum.y / as)
+                           : (ab > 0 ? make_double2(bigv.x / ab, -bigv.y / ab) : make_double2(1, 0));
+    }
+    __syncthreads();
+    double2 *Ub = U + b * n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int r = e / n, jj = e % n;
+        const int c = ord[jj];
+        const double s = nrm[c];
+        const double2 v = zmul(A[c * n + r], phs[c]);
+        Ub[r * n + jj] = s > 0 ? make_double2(v.x / s, v.y / s) : make_double2(0, 0);
+    }
+    for (int jj = threadIdx.x; jj < n; jj += blockDim.x) sig[b * n + jj] = nrm[ord[jj]];
+}
+
+// ------------------------------------------------------------ shifted solves
+// W (column-major n x n, in global scratch) = shift*I - M_b; LU with partial
+// pivoting applied to the right-hand side as it proceeds, then back substitution.
+__global__ __launch_bounds__(256) void k_bshift_solve(int n, int nrhs, const double2 *__restrict__ M,
+                                                      const double2 *__restrict__ shift, const double2 *B,
+                                                      double2 *X, double2 *__restrict__ work, int *info)
+{
+    extern __shared__ double2 x[];             // n
+    __shared__ double s_val[256];
+    __shared__ int s_idx[256];
+    const int64_t sys = blockIdx.x;
+    const int64_t b = sys / nrhs;
+    const int m = (int)(sys % nrhs);
+    const double2 *Mb = M + b * n * n;
+    double2 *W = work + sys * (int64_t)n * n;
+    const double2 sh = shift[b * nrhs + m];
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        const double2 v = Mb[e];
+        double2 w = make_double2(-v.x, -v.y);
+        if (i == c) { w.x += sh.x; w.y += sh.y; }
+        W[c * n + i] = w;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = B[(b * n + i) * nrhs + m];
+    __syncthreads();
+    int bad = 0;
+    for (int j = 0; j < n; ++j) {
+        // pivot search in column j
+        double best = -1;
+        int bi = j;
+        for (int i = j + threadIdx.x; i < n; i += blockDim.x) {
+            const double a = zabs2(W[j * n + i]);
+            if (a > best) { best = a; bi = i; }
+        }
+        s_val[threadIdx.x] = best;
+        s_idx[threadIdx.x] = bi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                const double v2 = s_val[threadIdx.x + o];
+                const int i2 = s_idx[threadIdx.x + o];
+                if (v2 > s_val[threadIdx.x] || (v2 == s_val[threadIdx.x] && i2 < s_idx[threadIdx.x])) {
+                    s_val[threadIdx.x] = v2;
+                    s_idx[threadIdx.x] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = s_idx[0];
+        if (!(s_val[0] > 0)) bad = 1;
+        if (p != j) {
+            for (int c = j + threadIdx.x; c < n; c += blockDim.x) {
+                const double2 t = W[c * n + j];
+                W[c * n + j] = W[c * n + p];
+                W[c * n + p] = t;
+            }
+            if (threadIdx.x == 0) { const double2 t = x[j]; x[j] = x[p]; x[p] = t; }
+        }
+        __syncthreads();
+        const double2 d = W[j * n + j];
+        const double dd = zabs2(d);
+        const double2 dinv = dd > 0 ? make_double2(d.x / dd, -d.y / dd) : make_double2(0, 0);
+        // multipliers stored in column j below the diagonal
+        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) W[j * n + i] = zmul(W[j * n + i], dinv);
+        __syncthreads();
+        const int mm = n - j - 1;
+        for (int e = threadIdx.x; e < mm * mm; e += blockDim.x) {
+            const int c = j + 1 + e / mm, i = j + 1 + e % mm;
+            W[c * n + i] = zsub(W[c * n + i], zmul(W[j * n + i], W[c * n + j]));
+        }
+        if (threadIdx.x == 0) {
+            const double2 xj = x[j];
+            for (int i = j + 1; i < n; ++i) x[i] = zsub(x[i], zmul(W[j * n + i], xj));
+        }
+        __syncthreads();
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        if (threadIdx.x == 0) {
+            const double2 d = W[c * n + c];
+            const double dd = zabs2(d);
+            x[c] = dd > 0 ? zmul(x[c], make_double2(d.x / dd, -d.y / dd)) : make_double2(0, 0);
+        }
+        __syncthreads();
+        const double2 xc = x[c];
+        for (int i = threadIdx.x; i < c; i += blockDim.x) x[i] = zsub(x[i], zmul(W[c * n + i], xc));
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[(b * n + i) * nrhs + m] = x[i];
+    if (threadIdx.x == 0 && bad && info != nullptr) atomicAdd(info, 1);
+}
+
+}  // namespace rsvdt
+
+using namespace rsvdt;
+
+extern "C" int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel, double *R,
+                                   double *Rinv, int *info, rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
+    if (n > 1024 || nbatch > 0x7fffffff) { set_error_msg("rsvdt_hip_bchol_inv: n <= 1024"); return RSVDT_HIP_ERR_ARG; }
+    hipLaunchKernelGGL(k_bchol_inv, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n, (const double2 *)G,
+                       shift_rel, (double2 *)R, (double2 *)Rinv, info);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bchol_inv");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_btrmm_upper(int64_t nbatch, int64_t n, const double *Ra, const double *Rb, double *Rout,
+                                     rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
+    hipLaunchKernelGGL(k_btrmm_upper, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n,
+                       (const double2 *)Ra, (const double2 *)Rb, (double2 *)Rout);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_btrmm_upper");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_bjacobi_svd(int64_t nbatch, int64_t n, const double *R, double *U, double *sig,
+                                     rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
+    if (n > JAC_MAXN) { set_error_msg("rsvdt_hip_bjacobi_svd: n <= 64 required"); return RSVDT_HIP_ERR_ARG; }
+    hipLaunchKernelGGL(k_bjacobi_svd, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n,
+                       (const double2 *)R, (double2 *)U, sig);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bjacobi_svd");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" size_t rsvdt_hip_bshift_solve_work_bytes(int64_t nbatch, int64_t n, int64_t nrhs)
+{
+    return (size_t)(nbatch * nrhs) * (size_t)(n * n) * 16 + 256;
+}
+
+extern "C" int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs, const double *M, const double *shift,
+                                      const double *B, double *X, void *work, int *info, rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || n <= 0 || nrhs <= 0) return RSVDT_HIP_OK;
+    if (n > 4096 || nbatch * nrhs > 0x7fffffffLL) { set_error_msg("rsvdt_hip_bshift_solve: bad size"); return RSVDT_HIP_ERR_ARG; }
+    hipLaunchKernelGGL(k_bshift_solve, dim3((unsigned)(nbatch * nrhs)), dim3(256), (size_t)n * 16, as_stream(s),
+                       (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift, (const double2 *)B,
+                       (double2 *)X, (double2 *)work, info);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bshift_solve");
+    return RSVDT_HIP_OK;
+}

@@ -1,0 +1,298 @@
+// Tall complex-FP64 products for gfx950:
+//   * coef_gemm: out[p][l] = alpha * sum_q coef(q,p) * in[q][l] -- the forcing
+//     synthesis of SourceCode/CreateForcingOnFly.c:17-19 batched over a chunk
+//     of half-step samples and over all k test vectors, the snapshot DFT of
+//     SourceCode/DFT.c:26-38, and the windowed DFTs of SourceCode/GalerkinProj.c:30-46.
+//     L (= N*k complex columns) is huge and K, P <= ~130, so every workgroup
+//     streams a slab of `in` once and keeps a P-tile of accumulators per lane;
+//     the coefficient tile is wave-uniform and is read through the scalar cache.
+//   * bgram: batched conj(X)^T Z over tall row ranges (Gram matrices of
+//     CholeskyQR, the projections V^H(.) of the transient removal), reduced
+//     deterministically in two passes.
+//   * bapply: batched row-local small products Out = alpha*Y + X*C (the
+//     triangular applies of CholeskyQR, Q*U of the SVD, y1 - V*a).
+#include "common.h"
+
+namespace rsvdt {
+
+// ----------------------------------------------------------------- coef_gemm
+// FP64 MFMA (v_mfma_f64_16x16x4f64).  A wave owns a strip of 16 columns l and
+// computes all T*16 rows p of the workgroup's row range for it; the complex
+// product is four real MFMAs per K-step of 4:
+//   Re += Cr*Xr + (-Ci)*Xi,   Im += Cr*Xi + Ci*Xr.
+// Operand layouts (16x16x4, wave64): A(16x4): lane i -> A[i%16][i/16],
+// B(4x16): lane i -> B[i/16][i%16], D(16x16): lane i holds D[4j+i/16][i%16], j < 4
+// (checked on gfx950 with a known-answer product).
+// The K x T*16 coefficient block of the workgroup is staged in LDS once and
+// reused for every 64-column tile the workgroup visits (grid-stride), so
+// `in` is streamed from HBM exactly once per row range.
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+template <int T, typename OV>
+__global__ __launch_bounds__(256) void k_coef_gemm(
+    int64_t P, int64_t K, int64_t L, const double2 *__restrict__ coef, int64_t cq, int64_t cp, int64_t p0,
+    int sgn, int64_t cmod, double alpha, const double2 *__restrict__ in, int64_t ldi, OV *__restrict__ out,
+    int64_t ldo, double beta)
+{
+    extern __shared__ double2 cs[];                 // [Kpad][T*16]
+    constexpr int PW = T * 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t pbase = (int64_t)blockIdx.y * PW;
+    const int64_t Kpad = (K + 3) & ~(int64_t)3;
+    for (int64_t e = threadIdx.x; e < Kpad * PW; e += 256) {
+        const int64_t q = e / PW, pp = e % PW, p = pbase + pp;
+        double2 v = make_double2(0, 0);
+        if (q < K && p < P) {
+            int64_t col = p0 + (int64_t)sgn * p;
+            if (col < 0 || col >= cmod) { col %= cmod; if (col < 0) col += cmod; }
+            v = coef[q * cq + col * cp];
+        }
+        cs[e] = v;
+    }
+    __syncthreads();
+    const int kq = lane >> 4, cl = lane & 15;
+    const int64_t ntile = (L + 63) / 64;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t l = tile * 64 + wv * 16 + cl;
+        const bool lok = l < L;
+        dvec4 ar[T], ai[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
+        const double2 *src = in + l;
+        const int nks = (int)(Kpad >> 2);
+#pragma unroll 2
+        for (int ks = 0; ks < nks; ++ks) {
+            const int64_t q0 = (int64_t)ks * 4;
+            const int64_t q = q0 + kq;
+            const double2 x = (lok && q < K) ? src[q * ldi] : make_double2(0, 0);
+            const double2 *crow = cs + (q0 + kq) * PW + cl;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const double2 c = crow[t * 16];
+                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.x, x.x, ar[t], 0, 0, 0);
+                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.x, x.y, ai[t], 0, 0, 0);
+                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-c.y, x.y, ar[t], 0, 0, 0);
+                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.y, x.x, ai[t], 0, 0, 0);
+            }
+        }
+        if (lok) {
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int64_t p = pbase + t * 16 + 4 * j + kq;
+                    if (p < P) {
+                        OV *o = out + p * ldo + l;
+                        double orr = alpha * ar[t][j], oi = alpha * ai[t][j];
+                        if (beta != 0.0) { OV v = *o; orr += beta * (double)v.x; oi += beta * (double)v.y; }
+                        OV w;
+                        w.x = orr; w.y = oi;
+                        *o = w;
+                    }
+                }
+        }
+    }
+}
+
+template <int T>
+static int launch_coef(int64_t P, int64_t K, int64_t L, const double *coef, int64_t cq, int64_t cp, int64_t p0,
+                       int sgn, int64_t cmod, double alpha, const double *in, int64_t ldi, void *out, int64_t ldo,
+                       int out_prec, double beta, hipStream_t s)
+{
+    const int64_t Kpad = (K + 3) & ~(int64_t)3;
+    const size_t lds = (size_t)Kpad * T * 16 * 16;
+    const int64_t ntile = (L + 63) / 64;
+    // enough workgroups to fill 256 CUs several times; each visits ntile/gx tiles
+    const int64_t per_cu = lds > 80 * 1024 ? 1 : (lds > 40 * 1024 ? 2 : 4);
+    int64_t gx = 256 * per_cu * 2;
+    if (gx > ntile) gx = ntile;
+    dim3 grid((unsigned)gx, ceil_div(P, T * 16));
+    if (out_prec == 64)
+        hipLaunchKernelGGL((k_coef_gemm<T, double2>), grid, dim3(256), lds, s, P, K, L, (const double2 *)coef, cq, cp,
+                           p0, sgn, cmod, alpha, (const double2 *)in, ldi, (double2 *)out, ldo, beta);
+    else
+        hipLaunchKernelGGL((k_coef_gemm<T, float2>), grid, dim3(256), lds, s, P, K, L, (const double2 *)coef, cq, cp,
+                           p0, sgn, cmod, alpha, (const double2 *)in, ldi, (float2 *)out, ldo, beta);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_coef_gemm");
+    return RSVDT_HIP_OK;
+}
+
+// ----------------------------------------------------------------- bgram
+#define BG_T 16
+__global__ __launch_bounds__(256) void k_bgram_part(
+    int64_t nrows, int64_t na, int64_t nc, int64_t chunk, int ntc,
+    const double2 *__restrict__ X, int64_t xb, int64_t xr, int64_t xa,
+    const double2 *__restrict__ Z, int64_t zb, int64_t zr, int64_t zc, double2 *__restrict__ part)
+{
+    __shared__ double2 sx[BG_T][BG_T + 1];
+    __shared__ double2 sz[BG_T][BG_T + 1];
+    const int64_t b = blockIdx.z;
+    const int64_t nch = gridDim.x;
+    const int tile = blockIdx.y;
+    const int64_t a0 = (int64_t)(tile / ntc) * BG_T, c0 = (int64_t)(tile % ntc) * BG_T;
+    const int ti = threadIdx.x / BG_T, tj = threadIdx.x % BG_T;
+    const int64_t r_begin = (int64_t)blockIdx.x * chunk;
+    int64_t r_end = r_begin + chunk;
+    if (r_end > nrows) r_end = nrows;
+    const double2 *Xb = X + b * xb;
+    const double2 *Zb = Z + b * zb;
+    double2 acc = make_double2(0, 0);
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += BG_T) {
+        const int64_t r = r0 + ti;
+        const int64_t a = a0 + tj, c = c0 + tj;
+        sx[ti][tj] = (r < r_end && a < na) ? Xb[r * xr + a * xa] : make_double2(0, 0);
+        sz[ti][tj] = (r < r_end && c < nc) ? Zb[r * zr + c * zc] : make_double2(0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < BG_T; ++rr) {
+            const double2 xv = sx[rr][ti], zv = sz[rr][tj];
+            acc.x += xv.x * zv.x + xv.y * zv.y;
+            acc.y += xv.x * zv.y - xv.y * zv.x;
+        }
+        __syncthreads();
+    }
+    const int64_t a = a0 + ti, c = c0 + tj;
+    if (a < na && c < nc) part[((b * nch + blockIdx.x) * na + a) * nc + c] = acc;
+}
+
+__global__ void k_bgram_reduce(int64_t nbatch, int64_t nch, int64_t na, int64_t nc,
+                               const double2 *__restrict__ part, double2 *__restrict__ G)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = na * nc;
+    if (i >= nbatch * per) return;
+    const int64_t b = i / per, e = i % per;
+    double2 s = make_double2(0, 0);
+    for (int64_t ch = 0; ch < nch; ++ch) {
+        const double2 v = part[(b * nch + ch) * per + e];
+        s.x += v.x; s.y += v.y;
+    }
+    G[i] = s;
+}
+
+static int64_t bgram_chunk(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc)
+{
+    // aim for ~2048 workgroups in flight in total, chunks of >= 256 rows
+    const int64_t tiles = ((na + BG_T - 1) / BG_T) * ((nc + BG_T - 1) / BG_T) * nbatch;
+    int64_t nch = (4096 + tiles - 1) / tiles;
+    if (nch < 1) nch = 1;
+    int64_t chunk = (nrows + nch - 1) / nch;
+    if (chunk < 256) chunk = 256;
+    chunk = (chunk + BG_T - 1) / BG_T * BG_T;
+    return chunk;
+}
+
+// ----------------------------------------------------------------- bapply
+__global__ __launch_bounds__(256) void k_bapply(
+    int64_t nrows, int64_t na, int64_t nc, int ncpad, int rb,
+    const double2 *X, int64_t xb, int64_t xr, int64_t xa, const double2 *__restrict__ C, double alpha,
+    const double2 *Y, int64_t yb, int64_t yr, int64_t yc, double2 *Out, int64_t ob, int64_t orow, int64_t oc)
+{
+    extern __shared__ double2 sxr[];          // rb x na
+    const int64_t b = blockIdx.y;
+    const int64_t r0 = (int64_t)blockIdx.x * rb;
+    const double2 *Xb = X + b * xb;
+    for (int i = threadIdx.x; i < rb * na; i += blockDim.x) {
+        const int rr = i / (int)na, a = i % (int)na;
+        const int64_t r = r0 + rr;
+        sxr[i] = (r < nrows) ? Xb[r * xr + a * xa] : make_double2(0, 0);
+    }
+    __syncthreads();
+    const int rr = threadIdx.x / ncpad, c = threadIdx.x % ncpad;
+    const int64_t r = r0 + rr;
+    if (rr >= rb || r >= nrows || c >= nc) return;
+    const double2 *Cb = C + b * na * nc;
+    double2 acc = make_double2(0, 0);
+    if (Y != nullptr) {
+        const double2 yv = Y[b * yb + r * yr + c * yc];
+        acc.x = alpha * yv.x; acc.y = alpha * yv.y;
+    }
+    const double2 *xrow = sxr + rr * na;
+    for (int64_t a = 0; a < na; ++a) {
+        const double2 xv = xrow[a], cv = Cb[a * nc + c];
+        acc.x += xv.x * cv.x - xv.y * cv.y;
+        acc.y += xv.x * cv.y + xv.y * cv.x;
+    }
+    Out[b * ob + r * orow + c * oc] = acc;
+}
+
+}  // namespace rsvdt
+
+using namespace rsvdt;
+
+extern "C" int rsvdt_hip_coef_gemm(int64_t P, int64_t K, int64_t L, const double *coef, int64_t cq, int64_t cp,
+                                   int64_t p0, int sgn, int64_t cmod, double alpha, const double *in, int64_t ldi,
+                                   void *out, int64_t ldo, int out_prec, double beta, rsvdt_stream_t s)
+{
+    if (P <= 0 || L <= 0) return RSVDT_HIP_OK;
+    if (K <= 0 || cmod <= 0 || (out_prec != 64 && out_prec != 32) || P > 65535LL * 16) {
+        set_error_msg("rsvdt_hip_coef_gemm: bad arguments");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(s);
+    const int64_t Kpad = (K + 3) & ~(int64_t)3;
+    if (Kpad * 16 * 16 * 2 > 150 * 1024) {
+        set_error_msg("rsvdt_hip_coef_gemm: K too large for the LDS coefficient block (K <= 296)");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    // widest row tile whose coefficient block fits in 144 KB of LDS
+    if (P > 64 && Kpad * 8 * 16 * 16 <= 144 * 1024)
+        return launch_coef<8>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
+    if (P > 32 && Kpad * 4 * 16 * 16 <= 144 * 1024)
+        return launch_coef<4>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
+    if (P > 16)
+        return launch_coef<2>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
+    return launch_coef<1>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
+}
+
+extern "C" size_t rsvdt_hip_bgram_work_bytes(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc)
+{
+    const int64_t chunk = bgram_chunk(nbatch, nrows, na, nc);
+    const int64_t nch = (nrows + chunk - 1) / chunk;
+    return (size_t)(nbatch * nch * na * nc) * 16 + 256;
+}
+
+extern "C" int rsvdt_hip_bgram(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc, const double *X, int64_t xb,
+                               int64_t xr, int64_t xa, const double *Z, int64_t zb, int64_t zr, int64_t zc, double *G,
+                               void *work, rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || na <= 0 || nc <= 0) return RSVDT_HIP_OK;
+    if (nrows <= 0) return rsvdt_hip_zero(G, (size_t)(nbatch * na * nc) * 16, s);
+    if (nbatch > 65535) { set_error_msg("rsvdt_hip_bgram: nbatch > 65535"); return RSVDT_HIP_ERR_ARG; }
+    const int64_t chunk = bgram_chunk(nbatch, nrows, na, nc);
+    const int64_t nch = (nrows + chunk - 1) / chunk;
+    const int nta = (int)((na + BG_T - 1) / BG_T), ntc = (int)((nc + BG_T - 1) / BG_T);
+    hipStream_t st = as_stream(s);
+    hipLaunchKernelGGL(k_bgram_part, dim3((unsigned)nch, (unsigned)(nta * ntc), (unsigned)nbatch), dim3(256), 0, st,
+                       nrows, na, nc, chunk, ntc, (const double2 *)X, xb, xr, xa, (const double2 *)Z, zb, zr, zc,
+                       (double2 *)work);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bgram(part)");
+    const int64_t tot = nbatch * na * nc;
+    hipLaunchKernelGGL(k_bgram_reduce, dim3(ceil_div(tot, 256)), dim3(256), 0, st, nbatch, nch, na, nc,
+                       (const double2 *)work, (double2 *)G);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bgram(reduce)");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_bapply(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc, const double *X, int64_t xb,
+                                int64_t xr, int64_t xa, const double *C, double alpha, const double *Y, int64_t yb,
+                                int64_t yr, int64_t yc, double *Out, int64_t ob, int64_t orow, int64_t oc,
+                                rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || nrows <= 0 || nc <= 0) return RSVDT_HIP_OK;
+    if (nc > 256 || na > 512 || nbatch > 65535) {
+        set_error_msg("rsvdt_hip_bapply: nc <= 256, na <= 512, nbatch <= 65535 required");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    int ncpad = 1;
+    while (ncpad < nc) ncpad <<= 1;
+    int rb = 256 / ncpad;
+    const int64_t rb_lds = 65536 / ((na > 0 ? na : 1) * 16);
+    if (rb > rb_lds) rb = (int)(rb_lds > 0 ? rb_lds : 1);
+    const size_t lds = (size_t)rb * (size_t)(na > 0 ? na : 1) * 16;
+    hipLaunchKernelGGL(k_bapply, dim3(ceil_div(nrows, rb), (unsigned)nbatch), dim3(256), lds, as_stream(s), nrows,
+                       na, nc, ncpad, rb, (const double2 *)X, xb, xr, xa, (const double2 *)C, alpha,
+                       (const double2 *)Y, yb, yr, yc, (double2 *)Out, ob, orow, oc);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bapply");
+    return RSVDT_HIP_OK;
+}

@@ -1,0 +1,271 @@
+// Fused CSR SpMM + classical-RK4 stage kernel for gfx950.
+//
+// Replaces the per-step sequence of SourceCode/TSRK4.c:18-47 (four
+// MatMultAdd, three VecWAXPY, four VecAXPY on one test vector at a time) by
+// four launches that each apply the operator to the whole N x k block of
+// test vectors and finish the stage's vector updates in the epilogue:
+//
+//   stage 1:  s = A y  + F0 ;  u' = y + (h/2) s ;  acc = y + (h/6) s
+//   stage 2:  s = A u' + Fh ;  u''= y + (h/2) s ;  acc = acc + (h/6*2) s
+//   stage 3:  s = A u''+ Fh ;  u' = y + h s     ;  acc = acc + (h/6*2) s
+//   stage 4:  s = A u' + F1 ;  y  = acc + (h/6) s  [+ FP64 snapshot copy]
+//
+// which is the reference's rounding order (y += h/6 k1; += h/6*2 k2; ...).
+// The accumulator starts from the forcing, as PETSc's MatMultAdd does.
+//
+// Layout: a block is N x k row-major, so a row gather reads k contiguous
+// complex values; LPR lanes (power of two >= k, capped at 64) cooperate on
+// one row and each lane owns CPL columns.  With xcd_remap the workgroups of
+// one XCD (dispatch is round-robin over the 8 XCDs) get a contiguous row
+// range, so the stencil neighbours a row gathers sit in that XCD's L2.
+#include "common.h"
+
+namespace rsvdt {
+
+template <typename T, bool AREAL> struct aval { using type = typename vec2<T>::type; };
+template <typename T> struct aval<T, true> { using type = T; };
+
+__device__ __forceinline__ double shfl_val(double v, int src, int w) { return __shfl(v, src, w); }
+__device__ __forceinline__ float shfl_val(float v, int src, int w) { return __shfl(v, src, w); }
+__device__ __forceinline__ double2 shfl_val(double2 v, int src, int w) {
+    return make_double2(__shfl(v.x, src, w), __shfl(v.y, src, w));
+}
+__device__ __forceinline__ float2 shfl_val(float2 v, int src, int w) {
+    return make_float2(__shfl(v.x, src, w), __shfl(v.y, src, w));
+}
+
+// Memory-level parallelism: the LPR lanes of a row first fetch up to 16 of
+// its (column, value) pairs with one coalesced load each, broadcast them with
+// cross-lane shuffles, then issue all the row's gathers back to back; the
+// row-local operands (forcing, y, acc) are loaded before the gathers.
+template <typename T, bool AREAL, int LPR, int CPL>
+__global__ __launch_bounds__(256) void k_rk4_stage(
+    int nrows, int k, int stage, int xcd_remap, int64_t bstride,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
+    const typename vec2<T>::type *__restrict__ uin, const typename vec2<T>::type *yv,
+    const typename vec2<T>::type *__restrict__ F, typename vec2<T>::type *__restrict__ acc,
+    typename vec2<T>::type *out, T cnext, T cacc, double2 *__restrict__ snap)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    constexpr int RPB = 256 / LPR;                  // rows per workgroup
+    constexpr int NB = 16;                          // nonzeros per batch
+    constexpr int EPL = LPR >= NB ? 1 : NB / LPR;   // entries each lane fetches per batch
+    int bx = blockIdx.x;
+    if (xcd_remap) {
+        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
+        bx = x * q + (x < r ? x : r) + idx;
+    }
+    const int lane = threadIdx.x & (LPR - 1);
+    const int row = bx * RPB + (int)(threadIdx.x / LPR);
+    if (row >= nrows) return;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const V *ub = uin + boff;
+    const AV *vals = (const AV *)vals_;
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+
+    // row-local operands first
+    V fr[CPL], yr[CPL], ar[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + t * LPR;
+        const int64_t idx = boff + (int64_t)row * k + j;
+        fr[t].x = 0; fr[t].y = 0; yr[t] = fr[t]; ar[t] = fr[t];
+        if (j < k) {
+            if (F != nullptr) fr[t] = F[idx];
+            if (stage != 4) yr[t] = yv[idx];
+            if (stage != 1) ar[t] = acc[idx];
+        }
+    }
+    T sr[CPL], si[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) { sr[t] = fr[t].x; si[t] = fr[t].y; }
+
+    for (int eb = e0; eb < e1; eb += NB) {
+        int mc[EPL];
+        AV mv[EPL];
+#pragma unroll
+        for (int u = 0; u < EPL; ++u) {
+            const int e = eb + lane + u * LPR;
+            const bool ok = (LPR <= NB || lane < NB) && e < e1;
+            mc[u] = ok ? colidx[e] : 0;
+            if (ok) mv[u] = vals[e];
+            else mv[u] = AV{};
+        }
+        const int cnt = (e1 - eb) < NB ? (e1 - eb) : NB;
+        int cs[NB];
+        AV as[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            cs[u] = __shfl(mc[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+            as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+        }
+        V xs[NB][CPL];
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                const int j = lane + t * LPR;
+                xs[u][t].x = 0; xs[u][t].y = 0;
+                if (u < cnt && j < k) xs[u][t] = ub[(int64_t)cs[u] * k + j];
+            }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            if (u < cnt) {
+#pragma unroll
+                for (int t = 0; t < CPL; ++t) {
+                    if constexpr (AREAL) {
+                        sr[t] += as[u] * xs[u][t].x;
+                        si[t] += as[u] * xs[u][t].y;
+                    } else {
+                        sr[t] += as[u].x * xs[u][t].x - as[u].y * xs[u][t].y;
+                        si[t] += as[u].x * xs[u][t].y + as[u].y * xs[u][t].x;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + t * LPR;
+        if (j >= k) continue;
+        const int64_t idx = boff + (int64_t)row * k + j;
+        if (stage == 4) {
+            V yn;
+            yn.x = ar[t].x + cacc * sr[t];
+            yn.y = ar[t].y + cacc * si[t];
+            out[idx] = yn;
+            if (snap != nullptr) snap[idx] = make_double2((double)yn.x, (double)yn.y);
+        } else {
+            V u;
+            u.x = yr[t].x + cnext * sr[t];
+            u.y = yr[t].y + cnext * si[t];
+            out[idx] = u;
+            V a;
+            if (stage == 1) {
+                a.x = yr[t].x + cacc * sr[t];
+                a.y = yr[t].y + cacc * si[t];
+            } else {
+                a.x = ar[t].x + cacc * sr[t];
+                a.y = ar[t].y + cacc * si[t];
+            }
+            acc[idx] = a;
+        }
+    }
+}
+
+template <typename T, bool AREAL, int LPR, int CPL>
+static int launch_rk4(const rsvdt_spmm_desc *d, int stage, const void *uin, const void *y, const void *F,
+                      void *acc, void *out, double cn, double ca, double *snap, hipStream_t s)
+{
+    using V = typename vec2<T>::type;
+    constexpr int RPB = 256 / LPR;
+    const unsigned gx = ceil_div(d->nrows, RPB);
+    const unsigned gy = (unsigned)d->nb;
+    const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
+    hipLaunchKernelGGL((k_rk4_stage<T, AREAL, LPR, CPL>), dim3(gx, gy), dim3(256), 0, s,
+                       (int)d->nrows, (int)d->k, stage, remap, d->nrows * d->k, d->rowptr, d->col, d->val,
+                       (const V *)uin, (const V *)y, (const V *)F, (V *)acc, (V *)out, (T)cn, (T)ca,
+                       (double2 *)snap);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_rk4_stage");
+    return RSVDT_HIP_OK;
+}
+
+template <typename T, bool AREAL>
+static int dispatch_lanes(const rsvdt_spmm_desc *d, int stage, const void *uin, const void *y, const void *F,
+                          void *acc, void *out, double cn, double ca, double *snap, hipStream_t s)
+{
+    int lpr = d->lanes;
+    const int k = (int)d->k;
+    if (lpr == 0) {
+        lpr = 4;
+        while (lpr < k && lpr < 64) lpr <<= 1;
+    }
+    const int cpl = (k + lpr - 1) / lpr;
+#define RSVDT_CASE(L, C)                                                                            \
+    if (lpr == L && cpl <= C) return launch_rk4<T, AREAL, L, C>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);
+    RSVDT_CASE(4, 1) RSVDT_CASE(8, 1) RSVDT_CASE(16, 1) RSVDT_CASE(32, 1) RSVDT_CASE(64, 1)
+    RSVDT_CASE(4, 2) RSVDT_CASE(8, 2) RSVDT_CASE(16, 2) RSVDT_CASE(32, 2) RSVDT_CASE(64, 2)
+    RSVDT_CASE(4, 4) RSVDT_CASE(8, 4) RSVDT_CASE(16, 4) RSVDT_CASE(32, 4) RSVDT_CASE(64, 4)
+#undef RSVDT_CASE
+    set_error_msg("rsvdt_hip_rk4_stage: unsupported k / lanes combination (k <= 256)");
+    return RSVDT_HIP_ERR_ARG;
+}
+
+}  // namespace rsvdt
+
+using namespace rsvdt;
+
+extern "C" int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage, const void *uin, const void *y,
+                                   const void *F, void *acc, void *out, double c_next, double c_acc,
+                                   double *snap, rsvdt_stream_t s)
+{
+    if (!d || d->nrows <= 0 || d->k <= 0 || d->nb <= 0 || stage < 1 || stage > 4) {
+        set_error_msg("rsvdt_hip_rk4_stage: bad descriptor or stage");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    if (d->nrows > 0x7fffffff || d->nb > 65535) {
+        set_error_msg("rsvdt_hip_rk4_stage: N or nb too large");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    if (d->lanes && (d->lanes & (d->lanes - 1))) {
+        set_error_msg("rsvdt_hip_rk4_stage: lanes must be a power of two");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(s);
+    if (d->prec == 64) {
+        if (d->a_real) return dispatch_lanes<double, true>(d, stage, uin, y, F, acc, out, c_next, c_acc, snap, st);
+        return dispatch_lanes<double, false>(d, stage, uin, y, F, acc, out, c_next, c_acc, snap, st);
+    }
+    if (d->prec == 32) {
+        if (d->a_real) return dispatch_lanes<float, true>(d, stage, uin, y, F, acc, out, c_next, c_acc, snap, st);
+        return dispatch_lanes<float, false>(d, stage, uin, y, F, acc, out, c_next, c_acc, snap, st);
+    }
+    set_error_msg("rsvdt_hip_rk4_stage: prec must be 64 or 32");
+    return RSVDT_HIP_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// Plain block SpMM Out_b = M * In_b for nb blocks (the weight and input/output
+// matrices of SourceCode/ApplyWeightMats.c:16-80 and SVDAllFreqs.c:63-64
+// applied to all frequencies at once).  M is rows x cols CSR, complex FP64;
+// In blocks are cols x k, Out blocks rows x k.
+namespace rsvdt {
+__global__ __launch_bounds__(256) void k_spmm_blocks(int nrows, int k, int lpr, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx, const double2 *__restrict__ val,
+                                                     const double2 *__restrict__ in, int64_t ibs,
+                                                     double2 *__restrict__ out, int64_t obs)
+{
+    const int rpb = 256 / lpr;
+    const int lane = threadIdx.x % lpr;
+    const int row = blockIdx.x * rpb + threadIdx.x / lpr;
+    if (row >= nrows) return;
+    const double2 *ib = in + (int64_t)blockIdx.y * ibs;
+    double2 *ob = out + (int64_t)blockIdx.y * obs;
+    for (int j = lane; j < k; j += lpr) {
+        double sr = 0, si = 0;
+        for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) {
+            const double2 a = val[e], x = ib[(int64_t)colidx[e] * k + j];
+            sr += a.x * x.x - a.y * x.y;
+            si += a.x * x.y + a.y * x.x;
+        }
+        ob[(int64_t)row * k + j] = make_double2(sr, si);
+    }
+}
+}  // namespace rsvdt
+
+extern "C" int rsvdt_hip_spmm_blocks(int64_t nrows, int64_t k, int64_t nb, const int32_t *rowptr, const int32_t *col,
+                                     const double *val_complex, const double *in, int64_t in_bstride, double *out,
+                                     int64_t out_bstride, rsvdt_stream_t s)
+{
+    if (nrows <= 0 || k <= 0 || nb <= 0) return RSVDT_HIP_OK;
+    if (nb > 65535 || nrows > 0x7fffffffLL) { set_error_msg("rsvdt_hip_spmm_blocks: bad size"); return RSVDT_HIP_ERR_ARG; }
+    int lpr = 1;
+    while (lpr < k && lpr < 64) lpr <<= 1;
+    const int rpb = 256 / lpr;
+    hipLaunchKernelGGL(k_spmm_blocks, dim3(ceil_div(nrows, rpb), (unsigned)nb), dim3(256), 0, as_stream(s), (int)nrows,
+                       (int)k, lpr, rowptr, col, (const double2 *)val_complex, (const double2 *)in, in_bstride,
+                       (double2 *)out, out_bstride);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_spmm_blocks");
+    return RSVDT_HIP_OK;
+}

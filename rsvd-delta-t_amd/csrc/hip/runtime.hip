@@ -1,0 +1,255 @@
+// Runtime part of the thin C ABI: device selection, memory, events, and the
+// small elementwise kernels (fills, conversions, strided copies, norms).
+#include <stdio.h>
+#include <string.h>
+#include "common.h"
+
+namespace rsvdt {
+static thread_local char g_err[512] = "";
+void set_error(const char *where, hipError_t e) {
+    snprintf(g_err, sizeof(g_err), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+}
+void set_error_msg(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+}  // namespace rsvdt
+
+using namespace rsvdt;
+
+extern "C" const char *rsvdt_hip_last_error(void) { return g_err; }
+
+extern "C" int rsvdt_hip_init(int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error_msg("rsvdt_hip_init: no HIP device visible");
+        return RSVDT_HIP_ERR_NODEV;
+    }
+    if (device < 0 || device >= n) {
+        set_error_msg("rsvdt_hip_init: device index out of range");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    RSVDT_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t p;
+    RSVDT_HIP_TRY(hipGetDeviceProperties(&p, device));
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+        char msg[256];
+        snprintf(msg, sizeof(msg), "rsvdt_hip_init: device is %s, this build targets gfx950 only", p.gcnArchName);
+        set_error_msg(msg);
+        return RSVDT_HIP_ERR_NODEV;
+    }
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_malloc(void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    RSVDT_HIP_TRY(hipMalloc(p, bytes));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_free(void *p) {
+    if (p) RSVDT_HIP_TRY(hipFree(p));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes) RSVDT_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes) RSVDT_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, rsvdt_stream_t s) {
+    if (bytes) RSVDT_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(s)));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_sync(void) {
+    RSVDT_HIP_TRY(hipDeviceSynchronize());
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_mem_info(size_t *f, size_t *t) {
+    RSVDT_HIP_TRY(hipMemGetInfo(f, t));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_event_create(void **ev) {
+    hipEvent_t e;
+    RSVDT_HIP_TRY(hipEventCreate(&e));
+    *ev = (void *)e;
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_event_destroy(void *ev) {
+    if (ev) RSVDT_HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_event_record(void *ev, rsvdt_stream_t s) {
+    RSVDT_HIP_TRY(hipEventRecord((hipEvent_t)ev, as_stream(s)));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_event_elapsed_ms(void *a, void *b, float *ms) {
+    RSVDT_HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+    RSVDT_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return RSVDT_HIP_OK;
+}
+
+// ---------------------------------------------------------------- fills
+__global__ void k_zero16(uint4 *p, int64_t n16) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+__global__ void k_zero1(unsigned char *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+extern "C" int rsvdt_hip_zero(void *p, size_t bytes, rsvdt_stream_t s) {
+    if (!bytes) return RSVDT_HIP_OK;
+    uintptr_t a = (uintptr_t)p;
+    if ((a & 15) == 0 && (bytes & 15) == 0) {
+        int64_t n16 = (int64_t)(bytes / 16);
+        unsigned g = (unsigned)((n16 + 255) / 256);
+        if (g > 65536u) g = 65536u;
+        hipLaunchKernelGGL(k_zero16, dim3(g), dim3(256), 0, as_stream(s), (uint4 *)p, n16);
+    } else {
+        hipLaunchKernelGGL(k_zero1, dim3(ceil_div((int64_t)bytes, 256)), dim3(256), 0, as_stream(s),
+                           (unsigned char *)p, (int64_t)bytes);
+    }
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_zero");
+    return RSVDT_HIP_OK;
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_zscale(int64_t n, double a, const double2 *x, double2 *y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) { double2 v = x[i]; y[i] = make_double2(a * v.x, a * v.y); }
+}
+extern "C" int rsvdt_hip_zscale(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s) {
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_zscale, dim3(g), dim3(256), 0, as_stream(s), n, a, (const double2 *)x, (double2 *)y);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_zscale");
+    return RSVDT_HIP_OK;
+}
+__global__ void k_z2c(int64_t n, const double2 *x, float2 *y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) { double2 v = x[i]; y[i] = make_float2((float)v.x, (float)v.y); }
+}
+__global__ void k_c2z(int64_t n, const float2 *x, double2 *y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) { float2 v = x[i]; y[i] = make_double2((double)v.x, (double)v.y); }
+}
+extern "C" int rsvdt_hip_z2c(int64_t n, const double *x, float *y, rsvdt_stream_t s) {
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_z2c, dim3(g), dim3(256), 0, as_stream(s), n, (const double2 *)x, (float2 *)y);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_z2c");
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_c2z(int64_t n, const float *x, double *y, rsvdt_stream_t s) {
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_c2z, dim3(g), dim3(256), 0, as_stream(s), n, (const float2 *)x, (double2 *)y);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_c2z");
+    return RSVDT_HIP_OK;
+}
+
+__global__ void k_zcopy3(int64_t nb, int64_t nr, int64_t nc, const double2 *src, int64_t sb, int64_t sr,
+                         int64_t sc, double2 *dst, int64_t db, int64_t dr, int64_t dc) {
+    int64_t n = nb * nr * nc;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        int64_t c = i % nc, t = i / nc, r = t % nr, b = t / nr;
+        dst[b * db + r * dr + c * dc] = src[b * sb + r * sr + c * sc];
+    }
+}
+extern "C" int rsvdt_hip_zcopy3(int64_t nb, int64_t nr, int64_t nc, const double *src, int64_t sb, int64_t sr,
+                                int64_t sc, double *dst, int64_t db, int64_t dr, int64_t dc, rsvdt_stream_t s) {
+    int64_t n = nb * nr * nc;
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_zcopy3, dim3(g), dim3(256), 0, as_stream(s), nb, nr, nc, (const double2 *)src, sb, sr,
+                       sc, (double2 *)dst, db, dr, dc);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_zcopy3");
+    return RSVDT_HIP_OK;
+}
+
+// sum |x|^2: per-block partials into work, then one block sums them (deterministic)
+__global__ void k_znorm2_part(int64_t n, const double2 *x, double *part) {
+    __shared__ double sh[256];
+    double acc = 0;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) acc += zabs2(x[i]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_sum_parts(int n, const double *part, double *out) {
+    __shared__ double sh[256];
+    double acc = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+extern "C" int rsvdt_hip_znorm2(int64_t n, const double *x, double *out, void *work, rsvdt_stream_t s) {
+    const int nblk = 1024;   // work must hold 1024 doubles
+    hipLaunchKernelGGL(k_znorm2_part, dim3(nblk), dim3(256), 0, as_stream(s), n, (const double2 *)x, (double *)work);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, as_stream(s), nblk, (const double *)work, out);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_znorm2");
+    return RSVDT_HIP_OK;
+}
+
+// ---------------------------------------------------------------- seeded sketch
+// Counter-based generator: entry (row r, logical column c) of a rows x cols
+// matrix stored in kb-column row-major blocks gets the two uniforms
+// u(seed, 2i), u(seed, 2i+1) with i = c*rows + r (its column-major index),
+// u(s, j) = (splitmix64(s*0x100000001B3 + j) >> 11 + 0.5) / 2^53, so the
+// sketch is independent of the layout and of the launch shape.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t j) {
+    return ((double)(mix64(seed * 0x100000001B3ULL + j) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__global__ void k_fill_random(int64_t rows, int64_t kb, int64_t nblk, int64_t col0, uint64_t seed, int gauss,
+                              double2 *out) {
+    const int64_t n = rows * kb * nblk;
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; p < n; p += stride) {
+        const int64_t b = p / (rows * kb), rem = p % (rows * kb), r = rem / kb, j = rem % kb;
+        const uint64_t i = (uint64_t)((b * kb + j + col0) * rows + r);
+        const double a = u01(seed, 2 * i), c = u01(seed, 2 * i + 1);
+        if (gauss) {
+            const double rad = sqrt(-2.0 * log(a)) * 0.70710678118654752440;
+            double sn, cs;
+            sincos(6.28318530717958647692 * c, &sn, &cs);
+            out[p] = make_double2(rad * cs, rad * sn);
+        } else {
+            out[p] = make_double2(a, c);
+        }
+    }
+}
+extern "C" int rsvdt_hip_fill_random(int64_t rows, int64_t kb, int64_t nblk, uint64_t seed, int gauss, double *out,
+                                     rsvdt_stream_t s) {
+    const int64_t n = rows * kb * nblk;
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_fill_random, dim3(g), dim3(256), 0, as_stream(s), rows, kb, nblk, (int64_t)0, (uint64_t)seed,
+                       gauss, (double2 *)out);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_fill_random");
+    return RSVDT_HIP_OK;
+}

@@ -1,0 +1,720 @@
+/*
+ * The hot path: resolvent actions by RK4 time stepping (TSActionRK4 and its
+ * callers), the snapshot DFT, the Galerkin transient removal and the power
+ * iteration.  The reference steps one test vector at a time with four SpMVs,
+ * two forcing GEMVs and seven AXPYs per step (SourceCode/TSActionRK4.c:89-122,
+ * TSRK4.c:18-47); here every step is four fused SpMM+stage launches over the
+ * whole N x k block, and the forcing of a chunk of steps is synthesized by one
+ * small-coefficient GEMM.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "rsvdt_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_D 3.14159265358979323846264338327950288
+
+/* scratch slots */
+enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP, SCR_WORK,
+       SCR_INFO, SCR_SIG, SCR_MISC, SCR_MISC2 };
+
+/* ------------------------------------------------------------------ stepper */
+typedef struct {
+    Mat A;
+    int dir, prec;
+    double h;
+    PetscInt N, k, Nwe, twoNs, S;
+    rsvdt_spmm_desc desc;
+    void *y, *u1, *u2, *acc, *fbuf;
+    size_t esz;                  /* bytes per complex entry of the state */
+    const double *Fhat;          /* Nwe blocks, FP64 */
+    const double *idft;          /* [2Ns][Nwe] FP64 complex */
+} stepper;
+
+static int pick_lanes(Mat A, int prec, int dir)
+{
+    if (g_rsvdt.opt.spmm_lanes) return g_rsvdt.opt.spmm_lanes;
+    return A->tuned_lanes[prec == 32][dir ? 0 : 1];
+}
+
+static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, PetscInt nb)
+{
+    memset(d, 0, sizeof(*d));
+    d->nrows = A->rows;
+    d->k = k;
+    d->nb = nb;
+    d->rowptr = dir ? A->d_rowptr : A->d_rowptrH;
+    d->col = dir ? A->d_col : A->d_colH;
+    d->val = dir ? (prec == 64 ? A->d_val64 : A->d_val32) : (prec == 64 ? A->d_valH64 : A->d_valH32);
+    d->a_real = A->is_real;
+    d->prec = prec;
+    d->lanes = pick_lanes(A, prec, dir);
+    d->xcd_remap = g_rsvdt.opt.xcd_remap;
+}
+
+static PetscInt auto_chunk(PetscInt N, PetscInt k, size_t esz)
+{
+    if (g_rsvdt.opt.chunk_steps > 0) return g_rsvdt.opt.chunk_steps;
+    size_t fr = 0, tot = 0;
+    rsvdt_hip_mem_info(&fr, &tot);
+    double blk = (double)N * (double)k * (double)esz;
+    PetscInt S = 32;
+    while (S > 4 && (2.0 * S + 1) * blk > 0.15 * (double)fr) S /= 2;
+    return S;
+}
+
+static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double h, PetscInt k, PetscInt Nwe,
+                                   PetscInt Ns, const double *Fhat, const double *idft)
+{
+    memset(st, 0, sizeof(*st));
+    st->A = A; st->dir = dir; st->prec = prec; st->h = h;
+    st->N = A->rows; st->k = k; st->Nwe = Nwe; st->twoNs = 2 * Ns;
+    st->esz = prec == 64 ? 16 : 8;
+    st->Fhat = Fhat; st->idft = idft;
+    st->S = auto_chunk(st->N, k, st->esz);
+    desc_for(&st->desc, A, dir, prec, k, 1);
+    size_t blk = (size_t)st->N * (size_t)k * st->esz;
+    st->y = rsvdt_scratch(SCR_Y, blk);
+    st->u1 = rsvdt_scratch(SCR_U1, blk);
+    st->u2 = rsvdt_scratch(SCR_U2, blk);
+    st->acc = rsvdt_scratch(SCR_ACC, blk);
+    st->fbuf = rsvdt_scratch(SCR_FBUF, blk * (size_t)(2 * st->S + 1));
+    if (!st->y || !st->u1 || !st->u2 || !st->acc || !st->fbuf)
+        return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate the RK4 work buffers (%.2f GB each): %s",
+                         blk / 1e9, rsvdt_hip_last_error());
+    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st->y, blk, g_rsvdt.stream));
+    return 0;
+}
+
+static void *fslot(stepper *st, PetscInt j)
+{
+    return (char *)st->fbuf + (size_t)j * (size_t)st->N * (size_t)st->k * st->esz;
+}
+
+/* Forcing samples for steps i0 .. i0+steps-1 (1-based): slot j holds the
+ * half-step position 2(i0-1)+j, i.e. the iDFT sample of index +pos (direct)
+ * or -pos (adjoint) mod 2Ns (SourceCode/TSRK4.c:21-27).  Slot 0 is carried
+ * over from the previous chunk's last slot, as TSRK4 carries F3 into k1. */
+static PetscErrorCode stepper_forcing(stepper *st, PetscInt i0, PetscInt steps, int first)
+{
+    const int64_t L = st->N * st->k;
+    const int sgn = st->dir ? 1 : -1;
+    const int oprec = st->prec;
+    if (first) {
+        RSVDT_HIPCHK("CreateForcingOnFly",
+                     rsvdt_hip_coef_gemm(1, st->Nwe, L, st->idft, 1, st->Nwe, 0, 1, st->twoNs, 1.0, st->Fhat, L,
+                                         fslot(st, 0), L, oprec, 0.0, g_rsvdt.stream));
+    } else {
+        RSVDT_HIPCHK("CreateForcingOnFly",
+                     rsvdt_hip_memcpy_d2d(fslot(st, 0), fslot(st, 2 * st->S), (size_t)L * st->esz, g_rsvdt.stream));
+    }
+    const int64_t pos1 = 2 * (i0 - 1) + 1;
+    RSVDT_HIPCHK("CreateForcingOnFly",
+                 rsvdt_hip_coef_gemm(2 * steps, st->Nwe, L, st->idft, 1, st->Nwe, sgn * pos1, sgn, st->twoNs, 1.0,
+                                     st->Fhat, L, fslot(st, 1), L, oprec, 0.0, g_rsvdt.stream));
+    return 0;
+}
+
+/* One RK4 step (SourceCode/TSRK4.c:18-47) on the whole block, forcing from
+ * slots 2s, 2s+1, 2s+2 of the current chunk. */
+static PetscErrorCode stepper_step(stepper *st, PetscInt s, double *snap)
+{
+    const double h = st->h;
+    void *F0 = fslot(st, 2 * s), *Fh = fslot(st, 2 * s + 1), *F1 = fslot(st, 2 * s + 2);
+    rsvdt_stream_t q = g_rsvdt.stream;
+    RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 1, st->y, st->y, F0, st->acc, st->u1, h / 2, h / 6, NULL, q));
+    RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 2, st->u1, st->y, Fh, st->acc, st->u2, h / 2, h / 6 * 2, NULL, q));
+    RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 3, st->u2, st->y, Fh, st->acc, st->u1, h, h / 6 * 2, NULL, q));
+    RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 4, st->u1, NULL, F1, st->acc, st->y, 0.0, h / 6, snap, q));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ event timing */
+/* Pairs of HIP events recorded on the run's stream around each phase; the
+ * elapsed times are read once at the end so that timing never stalls the
+ * launch queue. */
+typedef struct { void **ev; int n, cap; double ms; } phase_timer;
+static void pt_begin(phase_timer *t)
+{
+    if (t->n + 2 > t->cap) {
+        int nc = t->cap ? 2 * t->cap : 64;
+        void **ne = realloc(t->ev, sizeof(void *) * (size_t)nc);
+        if (!ne) return;
+        for (int i = t->cap; i < nc; i++) ne[i] = NULL;
+        t->ev = ne; t->cap = nc;
+    }
+    if (!t->ev[t->n]) rsvdt_hip_event_create(&t->ev[t->n]);
+    if (!t->ev[t->n + 1]) rsvdt_hip_event_create(&t->ev[t->n + 1]);
+    rsvdt_hip_event_record(t->ev[t->n], g_rsvdt.stream);
+}
+static void pt_end(phase_timer *t)
+{
+    if (t->n + 2 > t->cap) return;
+    rsvdt_hip_event_record(t->ev[t->n + 1], g_rsvdt.stream);
+    t->n += 2;
+}
+static void pt_collect(phase_timer *t)
+{
+    double ms = 0;
+    for (int i = 0; i + 1 < t->n; i += 2) {
+        float x = 0;
+        rsvdt_hip_event_elapsed_ms(t->ev[i], t->ev[i + 1], &x);
+        ms += x;
+    }
+    t->ms = ms;
+}
+static void pt_free(phase_timer *t)
+{
+    for (int i = 0; i < t->cap; i++) if (t->ev[i]) rsvdt_hip_event_destroy(t->ev[i]);
+    free(t->ev);
+    t->ev = NULL; t->n = t->cap = 0;
+}
+
+/* ------------------------------------------------------------------ SpMM autotune */
+/* Picks the lanes-per-row of the RK4 stage kernel for this operator and
+ * precision by timing five stage-2 launches per candidate; the choice is
+ * kept on the Mat and reused by every later action. */
+static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
+{
+    int *slot = &A->tuned_lanes[prec == 32][dir ? 0 : 1];
+    if (*slot || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
+    int cand[6], nc = 0;
+    for (int l = 4; l <= 64; l <<= 1) if (l >= k / 4 && l <= 64 && (k + l - 1) / l <= 4) cand[nc++] = l;
+    if (nc <= 1) { *slot = nc ? cand[0] : 0; return 0; }
+    rsvdt_spmm_desc dsc;
+    desc_for(&dsc, A, dir, prec, k, 1);
+    void *e0, *e1;
+    rsvdt_hip_event_create(&e0); rsvdt_hip_event_create(&e1);
+    float best = 1e30f; int bestl = 0;
+    for (int i = 0; i < nc; i++) {
+        dsc.lanes = cand[i];
+        /* stage 2 with uin = a, y = b, no forcing, acc = d, out = c: all scratch */
+        int rc = rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
+        if (rc) continue;
+        rsvdt_hip_event_record(e0, g_rsvdt.stream);
+        for (int r = 0; r < 5; r++) rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
+        rsvdt_hip_event_record(e1, g_rsvdt.stream);
+        float ms = 0;
+        rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+        if (ms < best) { best = ms; bestl = cand[i]; }
+    }
+    rsvdt_hip_event_destroy(e0); rsvdt_hip_event_destroy(e1);
+    *slot = bestl;
+    if (g_rsvdt.opt.verbose) rsvdt_printf("[rsvdt] autotune: %d lanes per row (%.3f ms per stage)\n", bestl, best / 5);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ time stepping */
+/* TSActionRK4.c:38-130: steps all k test vectors together from y = 0 over
+ * rend = Nt + Ns (+RR) steps and stores Nstore snapshots, every RR steps
+ * after Nt, into Y_all (block s = snapshot column s of every test vector;
+ * the adjoint stores them reversed, SourceCode/SaveSnapshots.c:19-28). */
+static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat Fhat, Mat Yall, PetscInt max_steps,
+                                    phase_timer *t_rk4, phase_timer *t_frc)
+{
+    RSVDT_CHK(rsvdt_aij_upload(A));
+    const int dir = V->TS.DirAdj ? 1 : 0;
+    const int prec = g_rsvdt.opt.precision;
+    const PetscInt k = V->RSVD.k, Nwe = V->RSVD.Nw_eff, N = A->rows;
+    const PetscInt Nstore = V->TS.TransientRemoval ? V->RSVD.Nw + 1 : V->RSVD.Nw;
+    PetscInt rend = V->TS.Nt + V->TS.Ns + (V->TS.TransientRemoval ? V->TS.ResRatio : 0);
+    if (max_steps > 0 && max_steps < rend) rend = max_steps;
+    stepper st;
+    RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
+    if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes) {
+        RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.acc));
+        desc_for(&st.desc, A, dir, prec, k, 1);
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
+    }
+    int64_t launches = 0;
+    for (PetscInt i0 = 1; i0 <= rend; i0 += st.S) {
+        PetscInt steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
+        if (t_frc) pt_begin(t_frc);
+        RSVDT_CHK(stepper_forcing(&st, i0, steps, i0 == 1));
+        if (t_frc) pt_end(t_frc);
+        if (t_rk4) pt_begin(t_rk4);
+        for (PetscInt s = 0; s < steps; s++) {
+            PetscInt i = i0 + s;
+            double *snap = NULL;
+            if (Yall && i > V->TS.Nt && (i - V->TS.Nt) % V->TS.ResRatio == 0) {
+                PetscInt pos = (i - V->TS.Nt) / V->TS.ResRatio - 1;
+                pos %= Nstore;
+                if (!dir) pos = Nstore - 1 - pos;
+                snap = Yall->d + 2 * (size_t)pos * (size_t)N * (size_t)k;
+            }
+            RSVDT_CHK(stepper_step(&st, s, snap));
+            launches += 4;
+        }
+        if (t_rk4) pt_end(t_rk4);
+    }
+    g_rsvdt.stats.steps = rend;
+    g_rsvdt.stats.rk4_launches = launches;
+    return 0;
+}
+
+/* Homogeneous RK4 steps on nb blocks of N x k in place (SourceCode/TSTransRK4.c:15-36
+ * via TSExpADeltaT.c:25-34), FP64, in groups that bound the temporaries. */
+PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps, double *X, PetscInt N, PetscInt k,
+                                       PetscInt nb)
+{
+    RSVDT_CHK(rsvdt_aij_upload(A));
+    size_t blk = (size_t)N * (size_t)k * 16;
+    size_t fr = 0, tot = 0;
+    rsvdt_hip_mem_info(&fr, &tot);
+    PetscInt G = nb;
+    while (G > 1 && 3.0 * (double)G * (double)blk > 0.4 * (double)fr) G = (G + 1) / 2;
+    if (G > 65535) G = 65535;
+    double *u1 = rsvdt_scratch(SCR_U1, blk * (size_t)G);
+    double *u2 = rsvdt_scratch(SCR_U2, blk * (size_t)G);
+    double *acc = rsvdt_scratch(SCR_ACC, blk * (size_t)G);
+    if (!u1 || !u2 || !acc) return rsvdt_err(RSVDT_ERR_MEM, "TSExpADeltaT", "cannot allocate work buffers");
+    for (PetscInt g0 = 0; g0 < nb; g0 += G) {
+        PetscInt g = G < nb - g0 ? G : nb - g0;
+        rsvdt_spmm_desc d;
+        desc_for(&d, A, dir, 64, k, g);
+        d.lanes = 0;
+        double *y = X + 2 * (size_t)g0 * (size_t)N * (size_t)k;
+        rsvdt_stream_t q = g_rsvdt.stream;
+        for (PetscInt it = 0; it < nsteps; it++) {
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 1, y, y, NULL, acc, u1, h / 2, h / 6, NULL, q));
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 2, u1, y, NULL, acc, u2, h / 2, h / 6 * 2, NULL, q));
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 3, u2, y, NULL, acc, u1, h, h / 6 * 2, NULL, q));
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 4, u1, NULL, NULL, acc, y, 0.0, h / 6, NULL, q));
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ CholeskyQR */
+/* Shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 2020): a first
+ * pass on X^H X + s I with s = 11(mn + n(n+1)) u ||X||_F^2, then CholeskyQR2.
+ * Q is X R^{-1} with R = R3 R2 R1 upper triangular with a positive real
+ * diagonal -- the unique factor that SLEPc's Gram-Schmidt BVOrthogonalize
+ * also returns for a full-rank input (SourceCode/QRAllFreqs.c:26). */
+PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
+                            double *R_out)
+{
+    size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
+    double *G = rsvdt_scratch(SCR_G, mat);
+    double *R = rsvdt_scratch(SCR_R, mat);
+    double *Ri = rsvdt_scratch(SCR_RINV, mat);
+    double *Ra = rsvdt_scratch(SCR_RACC, mat);
+    double *Rt = rsvdt_scratch(SCR_RTMP, mat);
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(nb, nrows, n, n));
+    int *info = rsvdt_scratch(SCR_INFO, 256);
+    if (!G || !R || !Ri || !Ra || !Rt || !work || !info)
+        return rsvdt_err(RSVDT_ERR_MEM, "QRAllFreqs", "cannot allocate CholeskyQR work");
+    rsvdt_stream_t q = g_rsvdt.stream;
+    RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_zero(info, 256, q));
+    const double shift_rel = 11.0 * ((double)nrows * (double)n + (double)n * (double)(n + 1)) * 1.1102230246251565e-16;
+    for (int pass = 0; pass < 3; pass++) {
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, X, xb, xr, xa, X, xb, xr, xa, G, work, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, G, pass == 0 ? shift_rel : 0.0, R, Ri, info, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, X, xb, xr, xa, Ri, 0.0, NULL, 0, 0, 0, X, xb, xr, xa, q));
+        if (R_out) {
+            if (pass == 0) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, R, mat, q));
+            else {
+                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_btrmm_upper(nb, n, R, Ra, Rt, q));
+                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, Rt, mat, q));
+            }
+        }
+    }
+    if (R_out) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(R_out, Ra, mat, q));
+    int hinfo = 0;
+    RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)));
+    if (hinfo) rsvdt_printf("[rsvdt] warning: CholeskyQR met %d non-positive pivots (rank-deficient input)\n", hinfo);
+    return 0;
+}
+
+/* Left singular vectors (descending) of nb tall N x n matrices in place:
+ * CholeskyQR, one-sided Jacobi on R, Q <- Q U_R. */
+PetscErrorCode rsvdt_svd_left(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
+                              double *sig_dev)
+{
+    size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
+    double *Rf = rsvdt_scratch(SCR_MISC, mat);
+    double *U = rsvdt_scratch(SCR_MISC2, mat);
+    if (!Rf || !U) return rsvdt_err(RSVDT_ERR_MEM, "SVDAllFreqs", "cannot allocate SVD work");
+    RSVDT_CHK(rsvdt_cholqr(nb, nrows, n, X, xb, xr, xa, Rf));
+    RSVDT_HIPCHK("SVDAllFreqs", rsvdt_hip_bjacobi_svd(nb, n, Rf, U, sig_dev, g_rsvdt.stream));
+    RSVDT_HIPCHK("SVDAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, X, xb, xr, xa, U, 0.0, NULL, 0, 0, 0, X, xb, xr, xa,
+                                                 g_rsvdt.stream));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ Permute */
+/* SourceCode/PermuteMat.c:24-38 and ReversePermuteMat.c:24-38: column
+ * ik + iw*k <-> ik*Nw + iw.  The blocked layout makes both a relabelling. */
+PetscErrorCode PermuteMat(Mat F, RSVDt_vars *V)
+{
+    if (!F || F->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "PermuteMat", "dense Mat required");
+    PetscInt k = V->RSVD.k;
+    if (F->cols % k) return rsvdt_err(RSVDT_ERR_USER, "PermuteMat", "columns not a multiple of k");
+    if (F->order != 0 || F->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(F, k, 0));
+    F->order = 1;
+    return 0;
+}
+
+PetscErrorCode ReversePermuteMat(Mat F, RSVDt_vars *V)
+{
+    if (!F || F->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "ReversePermuteMat", "dense Mat required");
+    PetscInt k = V->RSVD.k;
+    if (F->cols % k) return rsvdt_err(RSVDT_ERR_USER, "ReversePermuteMat", "columns not a multiple of k");
+    if (F->order != 1 || F->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(F, k, 1));
+    F->order = 0;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ DFT */
+/* SourceCode/DFT.c:18-39: Y_hat = RR * Y_all * dft (per test vector), then the
+ * reverse permutation to frequency-major: one GEMM over all test vectors
+ * whose output blocks are already frequency-major. */
+PetscErrorCode DFT(Mat Yall, DFT_matrices *D, RSVD_matrices *R, RSVDt_vars *V)
+{
+    PetscInt Nw = D->dft->rows, Nwe = V->RSVD.Nw_eff, k = Yall->cols / Nw, N = Yall->rows;
+    if (Yall->cols % Nw) return rsvdt_err(RSVDT_ERR_USER, "DFT", "Y_all columns not a multiple of Nw");
+    if (Yall->order != 1 || Yall->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(Yall, k, 1));
+    Mat Y;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, k * Nwe, k, 0, &Y));
+    const int64_t L = N * k;
+    RSVDT_HIPCHK("DFT", rsvdt_hip_coef_gemm(Nwe, Nw, L, D->dft->d, Nwe, 1, 0, 1, Nwe, (double)V->TS.ResRatio, Yall->d, L,
+                                            Y->d, L, 64, 0.0, g_rsvdt.stream));
+    RSVDT_MatDestroy(&Yall);
+    R->Y_hat = Y;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ transient removal */
+/* QR of the snapshot blocks, shared by CreateMtilde and GalerkinProjAllModes
+ * (the reference computes the identical factor twice,
+ * SourceCode/CreateMtilde.c:43-46 and GalerkinProjAllModes.c:31-33). */
+static Mat g_trV = NULL;
+static const void *g_trV_key = NULL;
+
+static PetscErrorCode snapshot_basis(Mat Yall, PetscInt k, Mat *Vout)
+{
+    if (g_trV && g_trV_key == (const void *)Yall->d) { *Vout = g_trV; return 0; }
+    RSVDT_MatDestroy(&g_trV);
+    PetscInt Ns1 = Yall->cols / k, N = Yall->rows;
+    if (Yall->order != 1 || Yall->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(Yall, k, 1));
+    RSVDT_CHK(rsvdt_mat_dup(Yall, &g_trV));
+    /* batch = test vector ik: X_ik(r, s) = V[s*N*k + r*k + ik] */
+    RSVDT_CHK(rsvdt_cholqr(k, N, Ns1, g_trV->d, 1, k, N * k, NULL));
+    g_trV_key = Yall->d;
+    *Vout = g_trV;
+    return 0;
+}
+
+/* SourceCode/CreateMtilde.c:41-54: M_tilde_ik = V_ik^H e^{A dT} V_ik for every
+ * test vector, e^{A dT} being RR homogeneous RK4 steps (TSExpADeltaT.c). */
+PetscErrorCode CreateMtilde(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR)
+{
+    PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1, N = Yall->rows;
+    Mat Vb;
+    RSVDT_CHK(snapshot_basis(Yall, k, &Vb));
+    Mat MV;
+    RSVDT_CHK(rsvdt_mat_dup(Vb, &MV));
+    RSVDT_CHK(rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, MV->d, N, k, Ns1));
+    RSVDT_CHK(rsvdt_mat_alloc_dense(Ns1, Ns1 * k, Ns1, 0, &TSR->M_tilde_all));
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(k, N, Ns1, Ns1));
+    if (!work) return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate work");
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(k, N, Ns1, Ns1, Vb->d, 1, k, N * k, MV->d, 1, k, N * k,
+                                                 TSR->M_tilde_all->d, work, g_rsvdt.stream));
+    RSVDT_MatDestroy(&MV);
+    return 0;
+}
+
+/* SourceCode/TSExpADeltaT.c:25-37: U <- e^{A dT} U column by column (RR
+ * homogeneous RK4 steps); here all columns at once.  U must hold whole N x k
+ * blocks (columns a multiple of k) in the snapshot layout, or a single N x c
+ * matrix (stepped as one block of c columns). */
+PetscErrorCode TSExpADeltaT(LNS_vars *LNS, RSVDt_vars *V, Mat U)
+{
+    if (U->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "TSExpADeltaT", "dense Mat required");
+    PetscInt kk = U->kb, nb = U->cols / U->kb;
+    if (U->kb == 1 && U->cols > 1) {              /* column-major: relayout to one row-major block */
+        RSVDT_CHK(rsvdt_mat_to_layout(U, U->cols, 0));
+        kk = U->cols; nb = 1;
+    }
+    return rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, U->d, U->rows, kk, nb);
+}
+
+/* SourceCode/QRDecomposition.c:9-27: orthonormalize the columns of V in place. */
+PetscErrorCode QRDecomposition(Mat Vm)
+{
+    if (Vm->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "QRDecomposition", "dense Mat required");
+    if (Vm->kb != Vm->cols) RSVDT_CHK(rsvdt_mat_to_layout(Vm, Vm->cols, 0));
+    return rsvdt_cholqr(1, Vm->rows, Vm->cols, Vm->d, 0, Vm->cols, 1, NULL);
+}
+
+/* SourceCode/GalerkinProjAllModes.c:28-42 with GalerkinProj.c:26-74, for all
+ * test vectors and frequencies at once:
+ *   y1 = RR*Y[:,W1]*dft_m, y2 = RR*Y[:,W2]*dft_m  (W1 = 0..Nw-1, W2 = 1..Nw,
+ *   swapped for the adjoint), ph = e^{+-i w m dT},
+ *   a  = (ph I - M_tilde)^{-1} V^H (ph y1 - y2),   Y_hat_m = y1 - V a.
+ * The two windowed DFTs are one GEMM producing y1 and t = ph*y1 - y2. */
+PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR, DFT_matrices *D)
+{
+    (void)LNS;
+    PetscInt k = V->RSVD.k, Nw = V->RSVD.Nw, Ns1 = Nw + 1, Nwe = V->RSVD.Nw_eff, N = Yall->rows;
+    int dir = V->TS.DirAdj ? 1 : 0;
+    double dT = V->TS.dt * (double)V->TS.ResRatio;
+    double RR = (double)V->TS.ResRatio;
+    Mat Vb;
+    RSVDT_CHK(snapshot_basis(Yall, k, &Vb));
+    /* host: dft (Nw x Nwe) and the window coefficients C[s][p], p < 2*Nwe */
+    double *dft = malloc(sizeof(double) * 2 * (size_t)(Nw * Nwe));
+    double *C = calloc((size_t)(Ns1 * 2 * Nwe) * 2, sizeof(double));
+    double *ph = malloc(sizeof(double) * 2 * (size_t)(k * Nwe));
+    if (!dft || !C || !ph) return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProjAllModes", "out of memory");
+    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_d2h(dft, D->dft->d, sizeof(double) * 2 * (size_t)(Nw * Nwe)));
+    const PetscInt w1 = dir ? 0 : 1, w2 = dir ? 1 : 0;
+    for (PetscInt m = 0; m < Nwe; m++) {
+        double ang = (dir ? 1.0 : -1.0) * V->RSVD.w * (double)m * dT;   /* GalerkinProj.c:49-50 */
+        double pr = cos(ang), pi = sin(ang);
+        for (PetscInt ik = 0; ik < k; ik++) { ph[2 * (ik * Nwe + m)] = pr; ph[2 * (ik * Nwe + m) + 1] = pi; }
+        for (PetscInt s = 0; s < Ns1; s++) {
+            double d1r = 0, d1i = 0, d2r = 0, d2i = 0;
+            PetscInt j1 = s - w1, j2 = s - w2;
+            if (j1 >= 0 && j1 < Nw) { d1r = dft[2 * (j1 * Nwe + m)]; d1i = dft[2 * (j1 * Nwe + m) + 1]; }
+            if (j2 >= 0 && j2 < Nw) { d2r = dft[2 * (j2 * Nwe + m)]; d2i = dft[2 * (j2 * Nwe + m) + 1]; }
+            double *c1 = C + 2 * (size_t)(s * 2 * Nwe + m);
+            double *ct = C + 2 * (size_t)(s * 2 * Nwe + Nwe + m);
+            c1[0] = d1r; c1[1] = d1i;
+            ct[0] = pr * d1r - pi * d1i - d2r;
+            ct[1] = pr * d1i + pi * d1r - d2i;
+        }
+    }
+    free(dft);
+    double *dC, *dph;
+    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_malloc((void **)&dC, sizeof(double) * 2 * (size_t)(Ns1 * 2 * Nwe)));
+    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_malloc((void **)&dph, sizeof(double) * 2 * (size_t)(k * Nwe)));
+    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_h2d(dC, C, sizeof(double) * 2 * (size_t)(Ns1 * 2 * Nwe)));
+    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_h2d(dph, ph, sizeof(double) * 2 * (size_t)(k * Nwe)));
+    free(C); free(ph);
+    /* y1 (blocks 0..Nwe-1) and t (blocks Nwe..2Nwe-1) */
+    Mat Y1T;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, 2 * Nwe * k, k, 0, &Y1T));
+    const int64_t L = N * k;
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_coef_gemm(2 * Nwe, Ns1, L, dC, 2 * Nwe, 1, 0, 1, 2 * Nwe, RR, Yall->d, L, Y1T->d,
+                                                     L, 64, 0.0, g_rsvdt.stream));
+    /* b_ik = V_ik^H t_ik  (k x Ns1 x Nwe) */
+    size_t nb_b = (size_t)k * (size_t)Ns1 * (size_t)Nwe * 16;
+    double *B = rsvdt_scratch(SCR_MISC, nb_b);
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(k, N, Ns1, Nwe));
+    void *lw = rsvdt_scratch(SCR_G, rsvdt_hip_bshift_solve_work_bytes(k, Ns1, Nwe));
+    int *info = rsvdt_scratch(SCR_INFO, 256);
+    if (!B || !work || !lw || !info) return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
+    const double *T = Y1T->d + 2 * (size_t)Nwe * (size_t)L;
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bgram(k, N, Ns1, Nwe, Vb->d, 1, k, L, T, 1, k, L, B, work, g_rsvdt.stream));
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_zero(info, 256, g_rsvdt.stream));
+    /* a = (ph I - M_tilde)^{-1} b, in place in B */
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bshift_solve(k, Ns1, Nwe, TSR->M_tilde_all->d, dph, B, B, lw, info,
+                                                        g_rsvdt.stream));
+    /* Y_hat(ik) = y1 - V a */
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_zscale((int64_t)k * Ns1 * Nwe, -1.0, B, B, g_rsvdt.stream));
+    Mat Yup;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Nwe * k, k, 1, &Yup));
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bapply(k, N, Ns1, Nwe, Vb->d, 1, k, L, B, 1.0, Y1T->d, 1, k, L, Yup->d, 1, k,
+                                                  L, g_rsvdt.stream));
+    int hinfo = 0;
+    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)));
+    if (hinfo) rsvdt_printf("[rsvdt] warning: %d singular Galerkin systems\n", hinfo);
+    rsvdt_hip_sync();
+    rsvdt_hip_free(dC); rsvdt_hip_free(dph);
+    RSVDT_MatDestroy(&Y1T);
+    RSVDT_MatDestroy(&TSR->M_tilde_all);      /* GalerkinProjAllModes.c:41 */
+    TSR->Y_hat_up = Yup;                       /* k-major, as the reference leaves it */
+    RSVDT_MatDestroy(&g_trV);
+    g_trV_key = NULL;
+    RSVDT_MatDestroy(&Yall);                   /* GalerkinProjAllModes.c:42 */
+    return 0;
+}
+
+/* SourceCode/TransientRemovalStrategy.c:30-40 */
+PetscErrorCode TransientRemovalStrategy(Mat Yall, LNS_vars *LNS, RSVD_matrices *R, RSVDt_vars *V,
+                                        TS_removal_matrices *TSR, DFT_matrices *D)
+{
+    double t1 = RSVDT_Time();
+    if (V->Display > 0)
+        rsvdt_printf("*** Trasnient removal strategy begins! ***\nThe process takes about as long as one period integration\n");
+    RSVDT_CHK(CreateMtilde(Yall, LNS, V, TSR));
+    RSVDT_CHK(GalerkinProjAllModes(Yall, LNS, V, TSR, D));
+    RSVDT_CHK(ReversePermuteMat(TSR->Y_hat_up, V));
+    R->Y_hat = TSR->Y_hat_up;
+    TSR->Y_hat_up = NULL;
+    if (V->Display > 0) rsvdt_printf("Trasnient removal strategy elapsed time = %.3f s\n", RSVDT_Time() - t1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ TSActionRK4 */
+PetscErrorCode TSActionRK4(RSVD_matrices *R, DFT_matrices *D, LNS_vars *LNS, RSVDt_vars *V, Directories *dirs,
+                           TS_removal_matrices *TSR)
+{
+    (void)dirs;
+    double t0 = RSVDT_Time();
+    if (V->Display > 0) rsvdt_printf(V->TS.DirAdj ? "\n*** Direct action begins! ***\n" : "\n*** Adjoint action begins! ***\n");
+    if (!R->Y_hat) return rsvdt_err(RSVDT_ERR_USER, "TSActionRK4", "Y_hat is not set");
+    PetscInt k = V->RSVD.k, N = V->RSVD.N, Nwe = V->RSVD.Nw_eff;
+    if (R->Y_hat->rows != N || R->Y_hat->cols != k * Nwe)
+        return rsvdt_err(RSVDT_ERR_USER, "TSActionRK4", "Y_hat must be %lld x %lld, got %lld x %lld", (long long)N,
+                         (long long)(k * Nwe), (long long)R->Y_hat->rows, (long long)R->Y_hat->cols);
+    memset(&g_rsvdt.stats, 0, sizeof(g_rsvdt.stats));
+    /* :49-50 Y_hat is moved into F_hat; the frequency-major blocks are what
+     * the forcing GEMM reads, so the permutation of :76 is not materialized */
+    R->F_hat = R->Y_hat;
+    R->Y_hat = NULL;
+    if (R->F_hat->order != 0 || R->F_hat->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(R->F_hat, k, 0));
+    PetscInt Nstore = V->TS.TransientRemoval ? V->RSVD.Nw + 1 : V->RSVD.Nw;
+    Mat Yall;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Nstore * k, k, 1, &Yall));
+    phase_timer trk = {0}, tfr = {0}, tdf = {0};
+    PetscErrorCode ierr = time_stepping(LNS->A, D, V, R->F_hat, Yall, 0, &trk, &tfr);
+    RSVDT_MatDestroy(&R->F_hat);                              /* :139 */
+    if (ierr) { RSVDT_MatDestroy(&Yall); return ierr; }
+    pt_begin(&tdf);
+    ierr = V->TS.TransientRemoval ? TransientRemovalStrategy(Yall, LNS, R, V, TSR, D) : DFT(Yall, D, R, V);
+    pt_end(&tdf);
+    pt_collect(&trk); pt_collect(&tfr); pt_collect(&tdf);
+    g_rsvdt.stats.rk4_ms = trk.ms;
+    g_rsvdt.stats.forcing_ms = tfr.ms;
+    g_rsvdt.stats.dft_ms = tdf.ms;
+    g_rsvdt.stats.tr_ms = V->TS.TransientRemoval ? tdf.ms : 0;
+    pt_free(&trk); pt_free(&tfr); pt_free(&tdf);
+    RSVDT_CHK(ierr);
+    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_sync());
+    g_rsvdt.stats.total_ms = 1e3 * (RSVDT_Time() - t0);
+    if (V->Display > 0)
+        rsvdt_printf("*** %s action elapsed time = %.3f s (RK4 %.1f ms, forcing %.1f ms, %s %.1f ms, %lld steps) ***\n",
+                     V->TS.DirAdj ? "Direct" : "Adjoint", RSVDT_Time() - t0, trk.ms, tfr.ms,
+                     V->TS.TransientRemoval ? "transient removal" : "DFT", tdf.ms, (long long)g_rsvdt.stats.steps);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ ApplyWeightMats */
+static PetscErrorCode apply_sparse(Mat M, int herm, Mat *Y, PetscInt k)
+{
+    RSVDT_CHK(rsvdt_aij_upload(M));
+    PetscInt rows = herm ? M->cols : M->rows, cols = herm ? M->rows : M->cols;
+    if ((*Y)->rows != cols)
+        return rsvdt_err(RSVDT_ERR_USER, "ApplyWeightMats", "size mismatch: matrix has %lld columns, Y_hat %lld rows",
+                         (long long)cols, (long long)(*Y)->rows);
+    if ((*Y)->order != 0 || (*Y)->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(*Y, k, 0));
+    PetscInt nb = (*Y)->cols / k;
+    Mat out;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(rows, (*Y)->cols, k, 0, &out));
+    /* complex FP64 values: the device copies hold real values when the matrix is real */
+    const void *val = herm ? M->d_valH64 : M->d_val64;
+    double *vz = (double *)val;
+    int own = 0;
+    if (M->is_real) {
+        /* expand the real values to complex for the block SpMM */
+        size_t nnz = (size_t)M->nnz;
+        double *h = malloc(sizeof(double) * 2 * (nnz ? nnz : 1));
+        double *hr = malloc(sizeof(double) * (nnz ? nnz : 1));
+        if (!h || !hr) return rsvdt_err(RSVDT_ERR_MEM, "ApplyWeightMats", "out of memory");
+        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_memcpy_d2h(hr, val, sizeof(double) * nnz));
+        for (size_t e = 0; e < nnz; e++) { h[2 * e] = hr[e]; h[2 * e + 1] = 0; }
+        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_malloc((void **)&vz, sizeof(double) * 2 * (nnz ? nnz : 1)));
+        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_memcpy_h2d(vz, h, sizeof(double) * 2 * nnz));
+        free(h); free(hr);
+        own = 1;
+    }
+    RSVDT_HIPCHK("ApplyWeightMats",
+                 rsvdt_hip_spmm_blocks(rows, k, nb, herm ? M->d_rowptrH : M->d_rowptr, herm ? M->d_colH : M->d_col, vz,
+                                       (*Y)->d, cols * k, out->d, rows * k, g_rsvdt.stream));
+    if (own) { rsvdt_hip_sync(); rsvdt_hip_free(vz); }
+    RSVDT_MatDestroy(Y);
+    *Y = out;
+    return 0;
+}
+
+/* SourceCode/ApplyWeightMats.c:16-80 (the Hermitian transposes use the
+ * conjugate-transposed device copies built at load instead of transposing
+ * the matrices in place). */
+PetscErrorCode ApplyWeightMats(RSVD_matrices *R, RSVDt_vars *V, Weight_matrices *W, PetscBool before)
+{
+    PetscInt k = V->RSVD.k;
+    if (!W) return 0;
+    if (V->TS.DirAdj) {
+        if (before) {
+            if (W->InvInputWeightFlg) RSVDT_CHK(apply_sparse(W->W_f_sqrt_inv, 0, &R->Y_hat, k));
+            if (W->InputMatrixFlg) RSVDT_CHK(apply_sparse(W->B, 0, &R->Y_hat, k));
+        } else {
+            if (W->OutputMatrixFlg) RSVDT_CHK(apply_sparse(W->C, 0, &R->Y_hat, k));
+            if (W->OutputWeightFlg) RSVDT_CHK(apply_sparse(W->W_q_sqrt, 0, &R->Y_hat, k));
+        }
+    } else {
+        if (before) {
+            if (W->OutputWeightFlg) RSVDT_CHK(apply_sparse(W->W_q_sqrt, 1, &R->Y_hat, k));
+            if (W->OutputMatrixFlg) RSVDT_CHK(apply_sparse(W->C, 1, &R->Y_hat, k));
+        } else {
+            if (W->InputMatrixFlg) RSVDT_CHK(apply_sparse(W->B, 1, &R->Y_hat, k));
+            if (W->InvInputWeightFlg) RSVDT_CHK(apply_sparse(W->W_f_sqrt_inv, 1, &R->Y_hat, k));
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ actions */
+/* SourceCode/DirectActionRK4.c:24-28 */
+PetscErrorCode DirectActionRK4(RSVD_matrices *R, RSVDt_vars *V, LNS_vars *LNS, DFT_matrices *D, Weight_matrices *W,
+                               Directories *dirs, TS_removal_matrices *TSR)
+{
+    V->TS.DirAdj = PETSC_TRUE;
+    RSVDT_CHK(ApplyWeightMats(R, V, W, PETSC_TRUE));
+    RSVDT_CHK(TSActionRK4(R, D, LNS, V, dirs, TSR));
+    RSVDT_CHK(ApplyWeightMats(R, V, W, PETSC_FALSE));
+    return 0;
+}
+
+/* SourceCode/AdjointActionRK4.c:24-28 */
+PetscErrorCode AdjointActionRK4(RSVD_matrices *R, RSVDt_vars *V, LNS_vars *LNS, DFT_matrices *D, Weight_matrices *W,
+                                Directories *dirs, TS_removal_matrices *TSR)
+{
+    V->TS.DirAdj = PETSC_FALSE;
+    RSVDT_CHK(ApplyWeightMats(R, V, W, PETSC_TRUE));
+    RSVDT_CHK(TSActionRK4(R, D, LNS, V, dirs, TSR));
+    RSVDT_CHK(ApplyWeightMats(R, V, W, PETSC_FALSE));
+    return 0;
+}
+
+/* SourceCode/PowerIterationRK4.c:20-29 */
+PetscErrorCode PowerIterationRK4(RSVD_matrices *R, RSVDt_vars *V, LNS_vars *LNS, DFT_matrices *D, Weight_matrices *W,
+                                 Directories *dirs, TS_removal_matrices *TSR)
+{
+    for (PetscInt iq = 0; iq < V->RSVD.q; iq++) {
+        if (V->Display > 0) rsvdt_printf("\n******** Inside power iteration, %d/%d *******\n", (int)iq + 1, (int)V->RSVD.q);
+        RSVDT_CHK(QRAllFreqs(R, V));
+        RSVDT_CHK(AdjointActionRK4(R, V, LNS, D, W, dirs, TSR));
+        RSVDT_CHK(QRAllFreqs(R, V));
+        RSVDT_CHK(DirectActionRK4(R, V, LNS, D, W, dirs, TSR));
+    }
+    if (V->RSVD.q > 0 && V->Display > 0) rsvdt_printf("\n******** Power iteration DONE! *************\n");
+    return 0;
+}
+
+/* ------------------------------------------------------------------ bench */
+PetscErrorCode RSVDT_BenchSteps(RSVD_matrices *R, RSVDt_vars *V, LNS_vars *LNS, DFT_matrices *D, PetscInt nsteps,
+                                PetscInt warmup, double *ms_out, double *rk4_ms_out, double *forcing_ms_out)
+{
+    if (!R->Y_hat) return rsvdt_err(RSVDT_ERR_USER, "RSVDT_BenchSteps", "Y_hat is not set");
+    Mat F = R->Y_hat;
+    PetscInt k = V->RSVD.k;
+    if (F->order != 0 || F->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(F, k, 0));
+    if (warmup > 0) RSVDT_CHK(time_stepping(LNS->A, D, V, F, NULL, warmup, NULL, NULL));
+    RSVDT_HIPCHK("RSVDT_BenchSteps", rsvdt_hip_sync());
+    phase_timer trk = {0}, tfr = {0}, tall = {0};
+    pt_begin(&tall);
+    PetscErrorCode ierr = time_stepping(LNS->A, D, V, F, NULL, nsteps, &trk, &tfr);
+    pt_end(&tall);
+    RSVDT_HIPCHK("RSVDT_BenchSteps", rsvdt_hip_sync());
+    pt_collect(&trk); pt_collect(&tfr); pt_collect(&tall);
+    if (ms_out) *ms_out = tall.ms;
+    if (rk4_ms_out) *rk4_ms_out = trk.ms;
+    if (forcing_ms_out) *forcing_ms_out = tfr.ms;
+    pt_free(&trk); pt_free(&tfr); pt_free(&tall);
+    return ierr;
+}

@@ -1,0 +1,109 @@
+/*
+ * Internal declarations of the C host layer: the Mat/Vec handles, the
+ * options table, the device-resident operator and the error helpers.
+ */
+#ifndef RSVDT_INTERNAL_H
+#define RSVDT_INTERNAL_H
+
+#include <stdio.h>
+#include <stdint.h>
+#include "rsvdt.h"
+#include "rsvdt_hip.h"
+
+/* ---- Mat --------------------------------------------------------------- */
+enum { RSVDT_MAT_DENSE = 1, RSVDT_MAT_AIJ = 2 };
+
+/* Dense storage: the columns form nb = cols/kb blocks of kb columns; block b
+ * is a rows x kb row-major slab at d + b*rows*kb (complex FP64).  order 0:
+ * logical column c is (block c/kb, column c%kb) -- the frequency-major
+ * sketch; order 1: logical column c = j*nb + b is (block b, column j) -- the
+ * k-major view the reference's PermuteMat produces (SourceCode/PermuteMat.c:24-38),
+ * and the layout of the snapshot store Y_all.  Permutations therefore only
+ * relabel the handle. */
+struct _p_RSVDT_Mat {
+    int kind;
+    PetscInt rows, cols;
+    /* dense */
+    PetscInt kb;
+    int order;
+    double *d;
+    /* AIJ (CSR): host copy (int64, complex) and device copies (int32 indices)
+     * of the matrix and of its conjugate transpose, FP64 and FP32 values. */
+    PetscInt nnz;
+    int is_real;             /* ||Im||_F < 1e-12: values kept real on the device */
+    int64_t *h_rowptr, *h_col;
+    double *h_val;           /* 2*nnz */
+    int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
+    void *d_val64, *d_val32, *d_valH64, *d_valH32;
+    int tuned_lanes[2][2];   /* autotuned lanes per row: [precision 64/32][direct/adjoint] */
+};
+
+struct _p_RSVDT_Vec {
+    PetscInt n;
+    double *d;
+};
+
+/* ---- global runtime ---------------------------------------------------- */
+typedef struct {
+    int initialized;
+    int device;
+    rsvdt_stream_t stream;
+    RSVDT_Options opt;
+    RSVDT_ActionStats stats;
+    /* learned-sketch columns for CreateRandomMat */
+    double *sketch_host;
+    PetscInt sketch_N, sketch_cols, sketch_Nw_eff;
+    char inputs_file[PETSC_MAX_PATH_LEN];
+} rsvdt_runtime;
+
+extern rsvdt_runtime g_rsvdt;
+
+PetscErrorCode rsvdt_ensure_device(void);
+
+/* ---- errors ------------------------------------------------------------ */
+PetscErrorCode rsvdt_err(PetscErrorCode code, const char *fn, const char *fmt, ...);
+#define RSVDT_CHK(x)                                  \
+    do {                                              \
+        PetscErrorCode _ierr = (x);                   \
+        if (_ierr) return _ierr;                      \
+    } while (0)
+#define RSVDT_HIPCHK(fn, x)                                                                   \
+    do {                                                                                      \
+        int _h = (x);                                                                         \
+        if (_h) return rsvdt_err(_h, fn, "device call failed: %s", rsvdt_hip_last_error()); \
+    } while (0)
+
+int rsvdt_printf(const char *fmt, ...);
+
+/* ---- options table ----------------------------------------------------- */
+int rsvdt_opt_get_string(const char *key, char *out, size_t n);   /* returns 1 if set */
+int rsvdt_opt_get_real(const char *key, double *out, int *ok);
+int rsvdt_opt_get_int(const char *key, PetscInt *out, int *ok);
+int rsvdt_opt_get_bool(const char *key, PetscBool *out, int *ok);
+PetscErrorCode rsvdt_yaml_insert_file(const char *path);
+
+/* ---- Mat helpers ------------------------------------------------------- */
+PetscErrorCode rsvdt_mat_alloc_dense(PetscInt rows, PetscInt cols, PetscInt kb, int order, Mat *out);
+PetscErrorCode rsvdt_mat_dup(Mat src, Mat *out);
+PetscErrorCode rsvdt_mat_to_layout(Mat m, PetscInt kb, int order);   /* physical relayout if needed */
+PetscErrorCode rsvdt_aij_upload(Mat m);
+PetscErrorCode rsvdt_aij_real_check(Mat m);
+
+/* ---- kernels of the action (rsvdt_action.c) ----------------------------- */
+/* Stepping of nb homogeneous N x k blocks by nsteps RK4 steps in place. */
+PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps, double *X, PetscInt N,
+                                       PetscInt k, PetscInt nb);
+/* Shifted CholeskyQR3 of nbatch tall N x n matrices given by strides, in
+ * place; R (nbatch x n x n, device) optional. */
+PetscErrorCode rsvdt_cholqr(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
+                            int64_t xa, double *R_out);
+/* Left singular vectors (descending) of nbatch tall matrices, in place, and
+ * their singular values (device, nbatch x n). */
+PetscErrorCode rsvdt_svd_left(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
+                              int64_t xa, double *sig_dev);
+
+/* device scratch that is reused across calls (grown on demand, freed at Finalize) */
+void *rsvdt_scratch(int slot, size_t bytes);
+void rsvdt_scratch_free_all(void);
+
+#endif

@@ -1,0 +1,96 @@
+"""Parity of the resolvent actions (DirectActionRK4 / AdjointActionRK4 with
+the snapshot DFT or the transient removal) between the HIP engine and the
+FP64 CPU oracle, on the same seeded inputs.  Tolerance: 1e-10 relative
+(FP64 state); the FP32-state form is checked at 2e-5."""
+import numpy as np
+import pytest
+
+import kat
+import oracle
+from conftest import crandn, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # N, Nw, complex, TR, TL, k
+    (64, 16, True, False, 3.0, 3),
+    (64, 16, True, True, 1.0, 3),
+    (64, 16, False, False, 3.0, 2),
+    (64, 16, False, True, 1.0, 2),
+    (50, 7, True, False, 2.0, 5),
+    (65, 64, True, True, 0.0, 16),     # Nstore = N: exact removal, 16 columns
+]
+
+
+@pytest.mark.parametrize("N,Nw,cplx,tr,TL,k", CASES)
+@pytest.mark.parametrize("direct", [True, False])
+def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct):
+    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct}", flush=True)
+    A = kat.gl_operator(N=N, L=20.0)
+    if not cplx:
+        A = A.real.astype(complex)
+    rng = np.random.default_rng(N * 7 + Nw)
+    o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
+    F = crandn(rng, N, k * o.Nw_eff)
+    Yo = o.action(F, direct=direct)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
+    assert (s.Ns, s.Nt, s.RR) == (o.Ns, o.Nt, o.RR)
+    s.set_sketch(F)
+    s.direct() if direct else s.adjoint()
+    Yg = s.get_Y()
+    s.close()
+    assert rel(Yg, Yo) < 1e-10
+
+
+def test_chunk_boundaries_and_lanes(pkg):
+    """Forcing-chunk sizes that do not divide the step count, and every lane width."""
+    A = kat.gl_operator(N=48, L=20.0)
+    rng = np.random.default_rng(11)
+    o = oracle.Oracle(A, k=4, q=0, Nw=8, w=0.3, TL=1.3, dt=0.02, transient_removal=False)
+    F = crandn(rng, 48, 4 * o.Nw_eff)
+    Yo = o.action(F)
+    for chunk, lanes in [(1, 4), (3, 8), (7, 16), (32, 32), (5, 64)]:
+        print(f"chunk={chunk} lanes={lanes}", flush=True)
+        s = pkg.Session(A, k=4, q=0, Nw=8, w=0.3, TL=1.3, dt=0.02, transient_removal=False, chunk_steps=chunk,
+                        spmm_lanes=lanes)
+        s.set_sketch(F)
+        s.direct()
+        assert rel(s.get_Y(), Yo) < 1e-10
+        s.close()
+    pkg.set_options(chunk_steps=0, spmm_lanes=0)
+
+
+def test_config0_ginzburg_landau(pkg):
+    """The CPU-runnable Ginzburg-Landau case: N = 500, k = 10, Nw = 64,
+    w = 0.05, dt = T/4096, TL = 10, transient removal on; one direct action."""
+    print("config0 GL N=500 k=10 Nw=64 TR", flush=True)
+    A = kat.gl_operator(N=500)
+    rng = np.random.default_rng(2024)
+    k, Nw, w, TL, dt = 10, 64, 0.05, 10.0, 0.03068
+    o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True)
+    assert o.Ns == 4096 and o.RR == 64
+    F = crandn(rng, 500, k * o.Nw_eff)
+    Yo = o.action(F)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True)
+    s.set_sketch(F)
+    s.direct()
+    Yg = s.get_Y()
+    s.close()
+    assert rel(Yg, Yo) < 1e-10
+
+
+def test_fp32_state_form(pkg):
+    """Complex-FP32 RK4 state and SpMM (DFT, QR, SVD and transient removal in FP64)."""
+    A = kat.gl_operator(N=64, L=20.0)
+    rng = np.random.default_rng(5)
+    o = oracle.Oracle(A, k=3, q=0, Nw=16, w=0.3, TL=1.0, dt=0.02, transient_removal=True)
+    F = crandn(rng, 64, 3 * o.Nw_eff)
+    Yo = o.action(F)
+    s = pkg.Session(A, k=3, q=0, Nw=16, w=0.3, TL=1.0, dt=0.02, transient_removal=True, precision=32)
+    s.set_sketch(F)
+    s.direct()
+    e = rel(s.get_Y(), Yo)
+    s.close()
+    pkg.set_options(precision=64)
+    print(f"fp32 relative error {e:.2e}")
+    assert e < 2e-5

@@ -1,0 +1,176 @@
+"""Kernel-level parity of the gfx950 HIP kernels, called through the C ABI
+(include/rsvdt_hip.h), against dense numpy FP64 references."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import kat
+from conftest import crandn, rel
+from gpu_util import Dev, init
+
+pytestmark = pytest.mark.gpu
+V = ctypes.c_void_p
+I64 = ctypes.c_int64
+
+
+@pytest.fixture(scope="module")
+def L(pkg):
+    return init(pkg)
+
+
+@pytest.mark.parametrize("P,K,Lc,p0,sgn,cmod,oprec", [(64, 64, 5000, 3, 1, 200, 64), (17, 9, 777, 5, -1, 40, 64),
+                                                     (1, 64, 300, 0, 1, 100, 64), (129, 129, 1000, 0, 1, 129, 64),
+                                                     (33, 16, 4096, 7, -1, 64, 32)])
+def test_coef_gemm(L, pkg, P, K, Lc, p0, sgn, cmod, oprec):
+    print(f"coef_gemm P={P} K={K} L={Lc} p0={p0} sgn={sgn} cmod={cmod} prec={oprec}", flush=True)
+    rng = np.random.default_rng(P + K)
+    coef = crandn(rng, K, cmod)          # coef[q][c], cp = 1, cq = cmod
+    X = crandn(rng, K, Lc)
+    cols = (p0 + sgn * np.arange(P)) % cmod
+    ref = 0.7 * (coef[:, cols].T @ X)
+    dc, dx = Dev(pkg, coef), Dev(pkg, X)
+    dout = Dev(pkg, nbytes=P * Lc * (16 if oprec == 64 else 8))
+    L.rsvdt_hip_coef_gemm.argtypes = [I64, I64, I64, V, I64, I64, I64, ctypes.c_int, I64, ctypes.c_double, V, I64,
+                                      V, I64, ctypes.c_int, ctypes.c_double, V]
+    rc = L.rsvdt_hip_coef_gemm(P, K, Lc, dc.p, cmod, 1, p0, sgn, cmod, 0.7, dx.p, Lc, dout.p, Lc, oprec, 0.0, None)
+    assert rc == 0
+    out = dout.get(np.complex128 if oprec == 64 else np.complex64, (P, Lc))
+    assert rel(out, ref) < (1e-14 if oprec == 64 else 1e-6)
+
+
+@pytest.mark.parametrize("nb,nr,na,nc,layout", [(4, 3000, 16, 16, "freq"), (3, 5000, 65, 65, "snap"),
+                                                (2, 100, 5, 33, "freq"), (7, 999, 10, 3, "snap")])
+def test_bgram_bapply(L, pkg, nb, nr, na, nc, layout):
+    print(f"bgram nb={nb} nr={nr} na={na} nc={nc} {layout}", flush=True)
+    rng = np.random.default_rng(na * nc)
+    if layout == "freq":   # X[b][r][a]
+        X = crandn(rng, nb, nr, na); Z = crandn(rng, nb, nr, nc)
+        xs = (nr * na, na, 1); zs = (nr * nc, nc, 1)
+        Xb = X; Zb = Z
+    else:                  # X[a][r][b]
+        X = crandn(rng, na, nr, nb); Z = crandn(rng, nc, nr, nb)
+        xs = (1, nb, nr * nb); zs = (1, nb, nr * nb)
+        Xb = X.transpose(2, 1, 0); Zb = Z.transpose(2, 1, 0)
+    G = np.einsum("bra,brc->bac", Xb.conj(), Zb)
+    L.rsvdt_hip_bgram_work_bytes.restype = ctypes.c_size_t
+    L.rsvdt_hip_bgram_work_bytes.argtypes = [I64] * 4
+    dX, dZ = Dev(pkg, X), Dev(pkg, Z)
+    dG = Dev(pkg, nbytes=nb * na * nc * 16)
+    dw = Dev(pkg, nbytes=L.rsvdt_hip_bgram_work_bytes(nb, nr, na, nc))
+    L.rsvdt_hip_bgram.argtypes = [I64] * 4 + [V, I64, I64, I64, V, I64, I64, I64, V, V, V]
+    assert L.rsvdt_hip_bgram(nb, nr, na, nc, dX.p, *xs, dZ.p, *zs, dG.p, dw.p, None) == 0
+    assert rel(dG.get(np.complex128, (nb, na, nc)), G) < 1e-13
+    # bapply: Out = 0.5*Z + X @ C, with C nb x na x nc
+    C = crandn(rng, nb, na, nc)
+    ref = 0.5 * Zb + np.einsum("bra,bac->brc", Xb, C)
+    dC = Dev(pkg, C)
+    dO = Dev(pkg, nbytes=Z.nbytes)
+    L.rsvdt_hip_bapply.argtypes = [I64] * 4 + [V, I64, I64, I64, V, ctypes.c_double, V, I64, I64, I64, V, I64, I64,
+                                               I64, V]
+    assert L.rsvdt_hip_bapply(nb, nr, na, nc, dX.p, *xs, dC.p, 0.5, dZ.p, *zs, dO.p, *zs, None) == 0
+    out = dO.get(np.complex128, Z.shape)
+    if layout != "freq":
+        out = out.transpose(2, 1, 0)
+    assert rel(out, ref) < 1e-13
+
+
+@pytest.mark.parametrize("nb,n", [(5, 16), (3, 65), (2, 129), (8, 1)])
+def test_chol_jacobi_solve(L, pkg, nb, n):
+    print(f"dense nb={nb} n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    X = crandn(rng, nb, 4 * n + 3, n)
+    G = np.einsum("bra,brc->bac", X.conj(), X)
+    dG = Dev(pkg, G)
+    dR, dRi = Dev(pkg, nbytes=G.nbytes), Dev(pkg, nbytes=G.nbytes)
+    info = Dev(pkg, np.zeros(4, dtype=np.int32))
+    L.rsvdt_hip_bchol_inv.argtypes = [I64, I64, V, ctypes.c_double, V, V, V, V]
+    assert L.rsvdt_hip_bchol_inv(nb, n, dG.p, 0.0, dR.p, dRi.p, info.p, None) == 0
+    R = dR.get(np.complex128, G.shape)
+    Ri = dRi.get(np.complex128, G.shape)
+    for b in range(nb):
+        assert np.allclose(np.tril(R[b], -1), 0)
+        assert rel(R[b].conj().T @ R[b], G[b]) < 1e-13
+        assert np.allclose(Ri[b] @ R[b], np.eye(n), atol=1e-10)
+        assert np.all(np.diag(R[b]).real > 0)
+    if n <= 64:
+        sig_dev, dU = Dev(pkg, nbytes=nb * n * 8), Dev(pkg, nbytes=G.nbytes)
+        L.rsvdt_hip_bjacobi_svd.argtypes = [I64, I64, V, V, V, V]
+        assert L.rsvdt_hip_bjacobi_svd(nb, n, dR.p, dU.p, sig_dev.p, None) == 0
+        U = dU.get(np.complex128, G.shape)
+        sg = sig_dev.get(np.float64, (nb, n))
+        for b in range(nb):
+            Ul, sl, _ = np.linalg.svd(R[b])
+            assert np.allclose(sg[b], sl, rtol=1e-12)
+            assert np.allclose(U[b].conj().T @ U[b], np.eye(n), atol=1e-12)
+            for j in range(n):
+                c = np.vdot(U[b][:, j], Ul[:, j])
+                assert abs(abs(c) - 1) < 1e-9
+    # shifted solves (ph I - M) x = b for nrhs shifts
+    nrhs = 3
+    M = crandn(rng, nb, n, n) / np.sqrt(n)
+    B = crandn(rng, nb, n, nrhs)
+    sh = np.exp(1j * rng.uniform(0, 6, size=(nb, nrhs))) * 3
+    dM, dB, dsh = Dev(pkg, M), Dev(pkg, B), Dev(pkg, sh)
+    L.rsvdt_hip_bshift_solve_work_bytes.restype = ctypes.c_size_t
+    L.rsvdt_hip_bshift_solve_work_bytes.argtypes = [I64] * 3
+    dw = Dev(pkg, nbytes=L.rsvdt_hip_bshift_solve_work_bytes(nb, n, nrhs))
+    L.rsvdt_hip_bshift_solve.argtypes = [I64, I64, I64, V, V, V, V, V, V, V]
+    assert L.rsvdt_hip_bshift_solve(nb, n, nrhs, dM.p, dsh.p, dB.p, dB.p, dw.p, info.p, None) == 0
+    Xs = dB.get(np.complex128, B.shape)
+    for b in range(nb):
+        for m in range(nrhs):
+            x = np.linalg.solve(sh[b, m] * np.eye(n) - M[b], B[b, :, m])
+            assert rel(Xs[b, :, m], x) < 1e-11
+
+
+def rk4_ref(A, y, F0, Fh, F1, h):
+    k1 = A @ y + F0
+    k2 = A @ (y + h / 2 * k1) + Fh
+    k3 = A @ (y + h / 2 * k2) + Fh
+    k4 = A @ (y + h * k3) + F1
+    return y + h / 6 * k1 + h / 3 * k2 + h / 3 * k3 + h / 6 * k4
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nrows", I64), ("k", I64), ("nb", I64), ("rowptr", V), ("col", V), ("val", V), ("a_real", ctypes.c_int),
+                ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int)]
+
+
+@pytest.mark.parametrize("k,real,prec,nb,lanes", [(1, False, 64, 1, 0), (3, True, 64, 1, 0), (10, False, 64, 1, 0),
+                                                  (16, True, 64, 1, 0), (16, False, 64, 1, 8), (32, False, 64, 3, 0),
+                                                  (40, True, 64, 1, 0), (100, False, 64, 1, 0), (16, True, 32, 1, 0),
+                                                  (10, False, 32, 2, 4)])
+def test_rk4_step(L, pkg, k, real, prec, nb, lanes):
+    print(f"rk4 k={k} real={real} prec={prec} nb={nb} lanes={lanes}", flush=True)
+    rng = np.random.default_rng(k + nb)
+    N = 3001
+    A = sp.random(N, N, density=5.0 / N, random_state=k, format="csr", dtype=np.float64)
+    if not real:
+        A = A + 1j * sp.random(N, N, density=3.0 / N, random_state=k + 1, format="csr")
+    A = (A + sp.eye(N) * (-1.0)).tocsr().astype(np.complex128)
+    A.sort_indices()
+    h = 0.05
+    y = crandn(rng, nb, N, k); F = crandn(rng, 3, nb, N, k)
+    ref = np.stack([rk4_ref(A, y[b], F[0, b], F[1, b], F[2, b], h) for b in range(nb)])
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rdt = np.float64 if prec == 64 else np.float32
+    vals = A.data.real.astype(rdt) if real else A.data.astype(cdt)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
+    dy = Dev(pkg, y.astype(cdt)); dF = [Dev(pkg, F[i].astype(cdt)) for i in range(3)]
+    du1, du2, dacc = (Dev(pkg, nbytes=y.astype(cdt).nbytes) for _ in range(3))
+    dsnap = Dev(pkg, nbytes=y.nbytes)
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, lanes, 1)
+    L.rsvdt_hip_rk4_stage.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, V, V, V, V, V, ctypes.c_double,
+                                      ctypes.c_double, V, V]
+    f = L.rsvdt_hip_rk4_stage
+    assert f(ctypes.byref(d), 1, dy.p, dy.p, dF[0].p, dacc.p, du1.p, h / 2, h / 6, None, None) == 0
+    assert f(ctypes.byref(d), 2, du1.p, dy.p, dF[1].p, dacc.p, du2.p, h / 2, h / 6 * 2, None, None) == 0
+    assert f(ctypes.byref(d), 3, du2.p, dy.p, dF[1].p, dacc.p, du1.p, h, h / 6 * 2, None, None) == 0
+    assert f(ctypes.byref(d), 4, du1.p, None, dF[2].p, dacc.p, dy.p, 0.0, h / 6, dsnap.p, None) == 0
+    out = dy.get(cdt, y.shape)
+    snap = dsnap.get(np.complex128, y.shape)
+    tol = 1e-13 if prec == 64 else 3e-6
+    assert rel(out, ref) < tol
+    assert np.array_equal(snap, out.astype(np.complex128))
