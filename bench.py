@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--no-overlap", action="store_true", help="forcing GEMM on the RK4 stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards test vectors "
@@ -84,10 +85,11 @@ def main():
     T = 2 * np.pi / w
     TL = T / 2
     s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, precision=args.precision,
-                    chunk_steps=args.chunk)
+                    chunk_steps=args.chunk, overlap_forcing=0 if args.no_overlap else 1, verbose=1)
     s.random_sketch(seed=1373, gaussian=True)
     ms, rk_ms, fr_ms = s.bench_steps(args.steps, warmup=args.warmup)
-    S = pkg.get_options().chunk_steps or 32
+    st = pkg.action_stats()
+    S = st["chunk_steps"]
     esz = 16 if args.precision == 64 else 8
     vsz = 8 if args.precision == 64 else 4
     V = N * k * esz
@@ -117,7 +119,9 @@ def main():
         "config": {"workload": "flow2d N=393216 (3 fields, 512x256 periodic, 11 nnz/row) RK4 time stepping, "
                                "k=16, Nw=128 (Nw_eff=64), transient removal, FP64" if args.precision == 64 else
                                "flow2d N=393216 RK4 time stepping, k=16, Nw=128, FP32 state",
-                   "N": N, "k": k, "Nw": Nw, "dt": dt, "chunk_steps": S, "precision": args.precision},
+                   "N": N, "k": k, "Nw": Nw, "dt": dt, "chunk_steps": S, "precision": args.precision,
+                   "spmm_shape": {"lanes_per_row": st["spmm_lanes"], "variant": st["spmm_variant"],
+                                  "xcd_remap": st["spmm_xcd_remap"]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": None,
                      "kernel": "k_rk4_stage", "bytes_per_launch": int(bytes_launch),

@@ -219,6 +219,7 @@ typedef struct {
     int use_graph;        /* capture each forcing chunk's RK4 steps in a hipGraph */
     int autotune;         /* time SpMM lane widths once per operator and keep the fastest */
     int verbose;          /* print phase timings */
+    int overlap_forcing;  /* compute the next forcing chunk on a second stream during the RK4 steps */
 } RSVDT_Options;
 PetscErrorCode RSVDT_GetOptions(RSVDT_Options *o);
 PetscErrorCode RSVDT_SetOptions(const RSVDT_Options *o);
@@ -232,6 +233,8 @@ typedef struct {
     double total_ms;       /* the whole action */
     int64_t steps;         /* RK4 steps taken */
     int64_t rk4_launches;  /* RK4 stage launches */
+    int chunk_steps;       /* RK4 steps per forcing chunk */
+    int spmm_lanes, spmm_variant, spmm_xcd_remap;   /* SpMM launch shape used */
 } RSVDT_ActionStats;
 PetscErrorCode RSVDT_GetActionStats(RSVDT_ActionStats *s);
 

@@ -41,12 +41,18 @@ int rsvdt_hip_init(int device);                       /* select device, check gf
 const char *rsvdt_hip_last_error(void);               /* text of the last failure */
 int rsvdt_hip_malloc(void **p, size_t bytes);
 int rsvdt_hip_free(void *p);
+/* synchronous; both wait for all queued device work first */
 int rsvdt_hip_memcpy_h2d(void *dst, const void *src_host, size_t bytes);
 int rsvdt_hip_memcpy_d2h(void *dst_host, const void *src, size_t bytes);
 int rsvdt_hip_memcpy_d2d(void *dst, const void *src, size_t bytes, rsvdt_stream_t s);
 int rsvdt_hip_zero(void *p, size_t bytes, rsvdt_stream_t s);  /* fill kernel, graph-safe */
 int rsvdt_hip_sync(void);
 int rsvdt_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* streams (non-blocking, default priority) and cross-stream ordering */
+int rsvdt_hip_stream_create(rsvdt_stream_t *s);
+int rsvdt_hip_stream_destroy(rsvdt_stream_t s);
+int rsvdt_hip_stream_wait_event(rsvdt_stream_t s, void *ev);
 
 /* events for timing on a given stream */
 int rsvdt_hip_event_create(void **ev);
@@ -76,6 +82,9 @@ typedef struct {
     int prec;
     int lanes;            /* 0 = default choice */
     int xcd_remap;        /* 1 = give each XCD a contiguous row range */
+    int variant;          /* 0: 16-nonzero gather batches; 1: + nontemporal row-local
+                             streams; 2: 8-nonzero batches; 3: 8 + nontemporal
+                             (variants apply when k <= lanes) */
 } rsvdt_spmm_desc;
 
 int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,
@@ -131,10 +140,17 @@ int rsvdt_hip_bapply(int64_t nbatch, int64_t nrows, int64_t na, int64_t nc,
 /* ---- batched small dense kernels (one workgroup per matrix) ----------- */
 /* Cholesky of (G_b + shift_b*I) (upper R, G = R^H R) then Rinv_b = R^-1.
  * G, R, Rinv: nbatch x n x n complex FP64 row-major.  shift_rel >= 0:
- * shift_b = shift_rel * trace(G_b).  info (device int, may be NULL)
- * receives the count of non-positive pivots (clamped).                  */
-int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel,
+ * shift_b = shift_rel * trace(G_b).  drop_tol > 0: a pivot below drop_tol
+ * times the largest diagonal entry of G_b marks its column as lying in the
+ * span of the previous ones; it gets a zero row in R and a zero column in
+ * R^-1 (so X R^-1 has a zero column there).  info (device int, may be NULL)
+ * receives the count of other non-positive pivots (clamped).             */
+int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel, double drop_tol,
                         double *R, double *Rinv, int *info, rsvdt_stream_t s);
+/* D_b = diag(d_j), d_j = 1 where R_b(j,j) is not positive (a column that a
+ * dropping Cholesky zeroed), else 0; *count (device int, may be NULL) is
+ * increased by the number of such columns.  R, D: nbatch x n x n.         */
+int rsvdt_hip_bdrop_mask(int64_t nbatch, int64_t n, const double *R, double *D, int *count, rsvdt_stream_t s);
 /* Rout_b = Ra_b * Rb_b for upper-triangular n x n (row-major). */
 int rsvdt_hip_btrmm_upper(int64_t nbatch, int64_t n, const double *Ra, const double *Rb,
                           double *Rout, rsvdt_stream_t s);
@@ -156,6 +172,8 @@ int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs,
 /* ---- elementwise ------------------------------------------------------ */
 /* y = a*x (complex entries, FP64), n entries */
 int rsvdt_hip_zscale(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s);
+/* y += a*x (complex entries, FP64, real a), n entries */
+int rsvdt_hip_zaxpy(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s);
 /* precision conversions of complex arrays */
 int rsvdt_hip_z2c(int64_t n, const double *x, float *y, rsvdt_stream_t s);
 int rsvdt_hip_c2z(int64_t n, const float *x, double *y, rsvdt_stream_t s);

@@ -90,12 +90,13 @@ class Directories(ctypes.Structure):
 
 class RSVDT_Options(ctypes.Structure):
     _fields_ = [("precision", _int), ("chunk_steps", _int), ("spmm_lanes", _int), ("xcd_remap", _int),
-                ("use_graph", _int), ("autotune", _int), ("verbose", _int)]
+                ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int)]
 
 
 class RSVDT_ActionStats(ctypes.Structure):
     _fields_ = [("rk4_ms", _dbl), ("forcing_ms", _dbl), ("dft_ms", _dbl), ("tr_ms", _dbl), ("total_ms", _dbl),
-                ("steps", _i64), ("rk4_launches", _i64)]
+                ("steps", _i64), ("rk4_launches", _i64), ("chunk_steps", _int), ("spmm_lanes", _int),
+                ("spmm_variant", _int), ("spmm_xcd_remap", _int)]
 
 
 # Every symbol the C headers (include/rsvdt.h, include/rsvdt_hip.h) declare.
@@ -113,10 +114,11 @@ ENTRY_POINTS = [
 HIP_ENTRY_POINTS = [
     "rsvdt_hip_init", "rsvdt_hip_last_error", "rsvdt_hip_malloc", "rsvdt_hip_free", "rsvdt_hip_memcpy_h2d",
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
+    "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
     "rsvdt_hip_rk4_stage", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
-    "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_btrmm_upper",
-    "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale",
+    "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
+    "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
     "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2",
 ]
 

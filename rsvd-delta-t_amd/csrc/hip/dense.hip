@@ -15,7 +15,8 @@ namespace rsvdt {
 
 // ------------------------------------------------------------ Cholesky + inverse
 __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restrict__ G, double shift_rel,
-                                                   double2 *__restrict__ R, double2 *__restrict__ Rinv, int *info)
+                                                   double drop_tol, double2 *__restrict__ R,
+                                                   double2 *__restrict__ Rinv, int *info)
 {
     const int64_t b = blockIdx.x;
     const double2 *Gb = G + b * n * n;
@@ -23,11 +24,13 @@ __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restr
     double2 *Xb = Rinv + b * n * n;
     __shared__ double s_shift;
     __shared__ double s_piv;
+    __shared__ double s_dmax;
     __shared__ int s_bad;
     if (threadIdx.x == 0) {
-        double tr = 0;
-        for (int i = 0; i < n; ++i) tr += Gb[i * n + i].x;
+        double tr = 0, dm = 0;
+        for (int i = 0; i < n; ++i) { tr += Gb[i * n + i].x; dm = fmax(dm, Gb[i * n + i].x); }
         s_shift = shift_rel * tr;
+        s_dmax = dm;
         s_bad = 0;
     }
     __syncthreads();
@@ -41,13 +44,17 @@ __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restr
     for (int j = 0; j < n; ++j) {
         if (threadIdx.x == 0) {
             double d = Rb[j * n + j].x;
-            const double tiny = 1e-300;
-            if (!(d > tiny)) { d = tiny; s_bad++; }
-            s_piv = sqrt(d);
+            if (drop_tol > 0 && !(d > drop_tol * s_dmax)) {
+                s_piv = 0;                      // column j lies in the span of columns < j: drop it
+            } else {
+                const double tiny = 1e-300;
+                if (!(d > tiny)) { d = tiny; s_bad++; }
+                s_piv = sqrt(d);
+            }
             Rb[j * n + j] = make_double2(s_piv, 0);
         }
         __syncthreads();
-        const double inv = 1.0 / s_piv;
+        const double inv = s_piv > 0 ? 1.0 / s_piv : 0.0;
         for (int c = j + 1 + threadIdx.x; c < n; c += blockDim.x) {
             double2 v = Rb[j * n + c];
             Rb[j * n + c] = make_double2(v.x * inv, v.y * inv);
@@ -68,18 +75,41 @@ __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restr
         __syncthreads();
     }
     // inverse of the upper-triangular factor, one column per thread
+    // (dropped columns have a zero row in R and a zero column in R^-1)
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         for (int i = c + 1; i < n; ++i) Xb[i * n + c] = make_double2(0, 0);
         const double rcc = Rb[c * n + c].x;
+        if (!(rcc > 0)) {
+            for (int i = 0; i <= c; ++i) Xb[i * n + c] = make_double2(0, 0);
+            continue;
+        }
         Xb[c * n + c] = make_double2(1.0 / rcc, 0);
         for (int i = c - 1; i >= 0; --i) {
+            const double rii = Rb[i * n + i].x;
+            if (!(rii > 0)) { Xb[i * n + c] = make_double2(0, 0); continue; }
             double2 s = make_double2(0, 0);
             for (int l = i + 1; l <= c; ++l) s = zadd(s, zmul(Rb[i * n + l], Xb[l * n + c]));
-            const double rii = Rb[i * n + i].x;
             Xb[i * n + c] = make_double2(-s.x / rii, -s.y / rii);
         }
     }
     if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
+}
+
+// D_b = diag(1 where R_b(j,j) is not positive, else 0): the columns that a
+// dropping CholeskyQR zeroed (their R has a zero pivot); count += their number.
+__global__ void k_bdrop_mask(int64_t tot, int n, const double2 *__restrict__ R, double2 *__restrict__ D, int *count)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int64_t b = e / ((int64_t)n * n);
+    const int rem = (int)(e % ((int64_t)n * n));
+    const int i = rem / n, c = rem % n;
+    double v = 0.0;
+    if (i == c && !(R[b * n * n + (int64_t)i * n + i].x > 0)) {
+        v = 1.0;
+        if (count != nullptr) atomicAdd(count, 1);
+    }
+    D[e] = make_double2(v, 0.0);
 }
 
 __global__ void k_btrmm_upper(int n, const double2 *__restrict__ A, const double2 *__restrict__ B,
@@ -298,14 +328,25 @@ __global__ __launch_bounds__(256) void k_bshift_solve(int n, int nrhs, const dou
 
 using namespace rsvdt;
 
-extern "C" int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel, double *R,
-                                   double *Rinv, int *info, rsvdt_stream_t s)
+extern "C" int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, double shift_rel, double drop_tol,
+                                   double *R, double *Rinv, int *info, rsvdt_stream_t s)
 {
     if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
     if (n > 1024 || nbatch > 0x7fffffff) { set_error_msg("rsvdt_hip_bchol_inv: n <= 1024"); return RSVDT_HIP_ERR_ARG; }
     hipLaunchKernelGGL(k_bchol_inv, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n, (const double2 *)G,
-                       shift_rel, (double2 *)R, (double2 *)Rinv, info);
+                       shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_bchol_inv");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_bdrop_mask(int64_t nbatch, int64_t n, const double *R, double *D, int *count,
+                                    rsvdt_stream_t s)
+{
+    if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
+    const int64_t tot = nbatch * n * n;
+    hipLaunchKernelGGL(k_bdrop_mask, dim3(ceil_div(tot, 256)), dim3(256), 0, as_stream(s), tot, (int)n,
+                       (const double2 *)R, (double2 *)D, count);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_bdrop_mask");
     return RSVDT_HIP_OK;
 }
 

@@ -60,7 +60,6 @@ __global__ __launch_bounds__(256) void k_coef_gemm(
         for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
         const double2 *src = in + l;
         const int nks = (int)(Kpad >> 2);
-#pragma unroll 2
         for (int ks = 0; ks < nks; ++ks) {
             const int64_t q0 = (int64_t)ks * 4;
             const int64_t q = q0 + kq;

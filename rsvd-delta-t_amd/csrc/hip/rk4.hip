@@ -38,7 +38,29 @@ __device__ __forceinline__ float2 shfl_val(float2 v, int src, int w) {
 // its (column, value) pairs with one coalesced load each, broadcast them with
 // cross-lane shuffles, then issue all the row's gathers back to back; the
 // row-local operands (forcing, y, acc) are loaded before the gathers.
-template <typename T, bool AREAL, int LPR, int CPL>
+typedef double ev2d __attribute__((ext_vector_type(2)));
+typedef float ev2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2 *p, bool nt) {
+    if (nt) { ev2d v = __builtin_nontemporal_load((const ev2d *)p); return make_double2(v.x, v.y); }
+    return *p;
+}
+__device__ __forceinline__ float2 ld_stream(const float2 *p, bool nt) {
+    if (nt) { ev2f v = __builtin_nontemporal_load((const ev2f *)p); return make_float2(v.x, v.y); }
+    return *p;
+}
+__device__ __forceinline__ void st_stream(double2 *p, double2 v, bool nt) {
+    if (nt) { ev2d w = {v.x, v.y}; __builtin_nontemporal_store(w, (ev2d *)p); }
+    else *p = v;
+}
+__device__ __forceinline__ void st_stream(float2 *p, float2 v, bool nt) {
+    if (nt) { ev2f w = {v.x, v.y}; __builtin_nontemporal_store(w, (ev2f *)p); }
+    else *p = v;
+}
+
+// NB: nonzeros gathered per batch (16, or 8 for fewer registers and more
+// waves); NT: nontemporal hints on the row-local streams, which keeps the
+// L2 for the gathered stage vector.
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
 __global__ __launch_bounds__(256) void k_rk4_stage(
     int nrows, int k, int stage, int xcd_remap, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
@@ -49,7 +71,6 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
     constexpr int RPB = 256 / LPR;                  // rows per workgroup
-    constexpr int NB = 16;                          // nonzeros per batch
     constexpr int EPL = LPR >= NB ? 1 : NB / LPR;   // entries each lane fetches per batch
     int bx = blockIdx.x;
     if (xcd_remap) {
@@ -72,9 +93,9 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
         const int64_t idx = boff + (int64_t)row * k + j;
         fr[t].x = 0; fr[t].y = 0; yr[t] = fr[t]; ar[t] = fr[t];
         if (j < k) {
-            if (F != nullptr) fr[t] = F[idx];
-            if (stage != 4) yr[t] = yv[idx];
-            if (stage != 1) ar[t] = acc[idx];
+            if (F != nullptr) fr[t] = ld_stream(F + idx, NT);
+            if (stage != 4) yr[t] = ld_stream(yv + idx, NT);
+            if (stage != 1) ar[t] = ld_stream(acc + idx, NT);
         }
     }
     T sr[CPL], si[CPL];
@@ -135,12 +156,12 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
             yn.x = ar[t].x + cacc * sr[t];
             yn.y = ar[t].y + cacc * si[t];
             out[idx] = yn;
-            if (snap != nullptr) snap[idx] = make_double2((double)yn.x, (double)yn.y);
+            if (snap != nullptr) st_stream(snap + idx, make_double2((double)yn.x, (double)yn.y), NT);
         } else {
             V u;
             u.x = yr[t].x + cnext * sr[t];
             u.y = yr[t].y + cnext * si[t];
-            out[idx] = u;
+            out[idx] = u;                     // gathered by the next stage: keep it cacheable
             V a;
             if (stage == 1) {
                 a.x = yr[t].x + cacc * sr[t];
@@ -149,12 +170,12 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
                 a.x = ar[t].x + cacc * sr[t];
                 a.y = ar[t].y + cacc * si[t];
             }
-            acc[idx] = a;
+            st_stream(acc + idx, a, NT);
         }
     }
 }
 
-template <typename T, bool AREAL, int LPR, int CPL>
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
 static int launch_rk4(const rsvdt_spmm_desc *d, int stage, const void *uin, const void *y, const void *F,
                       void *acc, void *out, double cn, double ca, double *snap, hipStream_t s)
 {
@@ -163,7 +184,7 @@ static int launch_rk4(const rsvdt_spmm_desc *d, int stage, const void *uin, cons
     const unsigned gx = ceil_div(d->nrows, RPB);
     const unsigned gy = (unsigned)d->nb;
     const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk4_stage<T, AREAL, LPR, CPL>), dim3(gx, gy), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_rk4_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s,
                        (int)d->nrows, (int)d->k, stage, remap, d->nrows * d->k, d->rowptr, d->col, d->val,
                        (const V *)uin, (const V *)y, (const V *)F, (V *)acc, (V *)out, (T)cn, (T)ca,
                        (double2 *)snap);
@@ -182,8 +203,14 @@ static int dispatch_lanes(const rsvdt_spmm_desc *d, int stage, const void *uin, 
         while (lpr < k && lpr < 64) lpr <<= 1;
     }
     const int cpl = (k + lpr - 1) / lpr;
-#define RSVDT_CASE(L, C)                                                                            \
-    if (lpr == L && cpl <= C) return launch_rk4<T, AREAL, L, C>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);
+    const int var = d->variant;
+#define RSVDT_CASE(L, C)                                                                                        \
+    if (lpr == L && cpl <= C) {                                                                                 \
+        if (C == 1 && var == 1) return launch_rk4<T, AREAL, L, C, 16, true>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);  \
+        if (C == 1 && var == 2) return launch_rk4<T, AREAL, L, C, 8, false>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);  \
+        if (C == 1 && var == 3) return launch_rk4<T, AREAL, L, C, 8, true>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);   \
+        return launch_rk4<T, AREAL, L, C, 16, false>(d, stage, uin, y, F, acc, out, cn, ca, snap, s);                         \
+    }
     RSVDT_CASE(4, 1) RSVDT_CASE(8, 1) RSVDT_CASE(16, 1) RSVDT_CASE(32, 1) RSVDT_CASE(64, 1)
     RSVDT_CASE(4, 2) RSVDT_CASE(8, 2) RSVDT_CASE(16, 2) RSVDT_CASE(32, 2) RSVDT_CASE(64, 2)
     RSVDT_CASE(4, 4) RSVDT_CASE(8, 4) RSVDT_CASE(16, 4) RSVDT_CASE(32, 4) RSVDT_CASE(64, 4)

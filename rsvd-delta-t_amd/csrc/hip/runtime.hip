@@ -49,11 +49,15 @@ extern "C" int rsvdt_hip_free(void *p) {
     if (p) RSVDT_HIP_TRY(hipFree(p));
     return RSVDT_HIP_OK;
 }
+// Host<->device copies first wait for the whole device: the engine's work runs
+// on non-blocking streams, which the null-stream hipMemcpy does not order with.
 extern "C" int rsvdt_hip_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    RSVDT_HIP_TRY(hipDeviceSynchronize());
     if (bytes) RSVDT_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return RSVDT_HIP_OK;
 }
 extern "C" int rsvdt_hip_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    RSVDT_HIP_TRY(hipDeviceSynchronize());
     if (bytes) RSVDT_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return RSVDT_HIP_OK;
 }
@@ -67,6 +71,20 @@ extern "C" int rsvdt_hip_sync(void) {
 }
 extern "C" int rsvdt_hip_mem_info(size_t *f, size_t *t) {
     RSVDT_HIP_TRY(hipMemGetInfo(f, t));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_stream_create(rsvdt_stream_t *s) {
+    hipStream_t h;
+    RSVDT_HIP_TRY(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));   // default priority
+    *s = (rsvdt_stream_t)h;
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_stream_destroy(rsvdt_stream_t s) {
+    if (s) RSVDT_HIP_TRY(hipStreamDestroy((hipStream_t)s));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_stream_wait_event(rsvdt_stream_t s, void *ev) {
+    RSVDT_HIP_TRY(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)ev, 0));
     return RSVDT_HIP_OK;
 }
 extern "C" int rsvdt_hip_event_create(void **ev) {
@@ -126,6 +144,21 @@ extern "C" int rsvdt_hip_zscale(int64_t n, double a, const double *x, double *y,
     unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
     hipLaunchKernelGGL(k_zscale, dim3(g), dim3(256), 0, as_stream(s), n, a, (const double2 *)x, (double2 *)y);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_zscale");
+    return RSVDT_HIP_OK;
+}
+__global__ void k_zaxpy(int64_t n, double a, const double2 *x, double2 *y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const double2 v = x[i], w = y[i];
+        y[i] = make_double2(w.x + a * v.x, w.y + a * v.y);
+    }
+}
+extern "C" int rsvdt_hip_zaxpy(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s) {
+    if (n <= 0) return RSVDT_HIP_OK;
+    unsigned g = ceil_div(n, 256); if (g > 65536u) g = 65536u;
+    hipLaunchKernelGGL(k_zaxpy, dim3(g), dim3(256), 0, as_stream(s), n, a, (const double2 *)x, (double2 *)y);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_zaxpy");
     return RSVDT_HIP_OK;
 }
 __global__ void k_z2c(int64_t n, const double2 *x, float2 *y) {

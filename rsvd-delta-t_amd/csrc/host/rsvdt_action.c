@@ -17,27 +17,27 @@
 #define PI_D 3.14159265358979323846264338327950288
 
 /* scratch slots */
-enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP, SCR_WORK,
-       SCR_INFO, SCR_SIG, SCR_MISC, SCR_MISC2 };
+enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_FBUF2, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP,
+       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_NSLOTS };
+_Static_assert(SCR_NSLOTS <= RSVDT_NSCRATCH, "scratch slot table too small");
 
 /* ------------------------------------------------------------------ stepper */
+/* Forcing chunks are double-buffered: while the RK4 steps of chunk c run on
+ * the main stream from buffer c%2, the forcing GEMM of chunk c+1 (MFMA-bound)
+ * fills the other buffer on the forcing stream, ordered by events. */
 typedef struct {
     Mat A;
     int dir, prec;
     double h;
     PetscInt N, k, Nwe, twoNs, S;
     rsvdt_spmm_desc desc;
-    void *y, *u1, *u2, *acc, *fbuf;
+    void *y, *u1, *u2, *acc, *fbuf[2];
+    void *ev_frc[2], *ev_rk[2];
     size_t esz;                  /* bytes per complex entry of the state */
     const double *Fhat;          /* Nwe blocks, FP64 */
     const double *idft;          /* [2Ns][Nwe] FP64 complex */
 } stepper;
 
-static int pick_lanes(Mat A, int prec, int dir)
-{
-    if (g_rsvdt.opt.spmm_lanes) return g_rsvdt.opt.spmm_lanes;
-    return A->tuned_lanes[prec == 32][dir ? 0 : 1];
-}
 
 static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, PetscInt nb)
 {
@@ -50,18 +50,31 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->val = dir ? (prec == 64 ? A->d_val64 : A->d_val32) : (prec == 64 ? A->d_valH64 : A->d_valH32);
     d->a_real = A->is_real;
     d->prec = prec;
-    d->lanes = pick_lanes(A, prec, dir);
-    d->xcd_remap = g_rsvdt.opt.xcd_remap;
+    const int p = prec == 32, q = dir ? 0 : 1;
+    if (A->tuned_ok[p][q] && nb == 1) {
+        d->lanes = A->tuned_lanes[p][q];
+        d->variant = A->tuned_variant[p][q];
+        d->xcd_remap = A->tuned_remap[p][q];
+    } else {
+        /* measured best on the 2D flow operator: 8-nonzero gather batches with
+         * nontemporal row-local streams, lanes = k */
+        d->lanes = 0;
+        d->variant = 3;
+        d->xcd_remap = g_rsvdt.opt.xcd_remap;
+    }
+    if (g_rsvdt.opt.spmm_lanes) d->lanes = g_rsvdt.opt.spmm_lanes;
 }
 
+/* Steps per chunk: 2S+1 samples per chunk (both chunk ends included) keeps the
+ * GEMM's row count at 63/31/15/7, just under the 64/32/16/8 MFMA row tiles. */
 static PetscInt auto_chunk(PetscInt N, PetscInt k, size_t esz)
 {
     if (g_rsvdt.opt.chunk_steps > 0) return g_rsvdt.opt.chunk_steps;
     size_t fr = 0, tot = 0;
     rsvdt_hip_mem_info(&fr, &tot);
     double blk = (double)N * (double)k * (double)esz;
-    PetscInt S = 32;
-    while (S > 4 && (2.0 * S + 1) * blk > 0.15 * (double)fr) S /= 2;
+    PetscInt S = 31;
+    while (S > 3 && 2.0 * (2.0 * S + 1) * blk > 0.2 * (double)fr) S = (S - 1) / 2;
     return S;
 }
 
@@ -80,49 +93,59 @@ static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double
     st->u1 = rsvdt_scratch(SCR_U1, blk);
     st->u2 = rsvdt_scratch(SCR_U2, blk);
     st->acc = rsvdt_scratch(SCR_ACC, blk);
-    st->fbuf = rsvdt_scratch(SCR_FBUF, blk * (size_t)(2 * st->S + 1));
-    if (!st->y || !st->u1 || !st->u2 || !st->acc || !st->fbuf)
+    st->fbuf[0] = rsvdt_scratch(SCR_FBUF, blk * (size_t)(2 * st->S + 1));
+    st->fbuf[1] = rsvdt_scratch(SCR_FBUF2, blk * (size_t)(2 * st->S + 1));
+    if (!st->y || !st->u1 || !st->u2 || !st->acc || !st->fbuf[0] || !st->fbuf[1])
         return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate the RK4 work buffers (%.2f GB each): %s",
                          blk / 1e9, rsvdt_hip_last_error());
+    for (int b = 0; b < 2; b++) {
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_create(&st->ev_frc[b]));
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_create(&st->ev_rk[b]));
+    }
     RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st->y, blk, g_rsvdt.stream));
     return 0;
 }
 
-static void *fslot(stepper *st, PetscInt j)
+static void stepper_free(stepper *st)
 {
-    return (char *)st->fbuf + (size_t)j * (size_t)st->N * (size_t)st->k * st->esz;
+    for (int b = 0; b < 2; b++) {
+        rsvdt_hip_event_destroy(st->ev_frc[b]);
+        rsvdt_hip_event_destroy(st->ev_rk[b]);
+        st->ev_frc[b] = st->ev_rk[b] = NULL;
+    }
 }
 
-/* Forcing samples for steps i0 .. i0+steps-1 (1-based): slot j holds the
- * half-step position 2(i0-1)+j, i.e. the iDFT sample of index +pos (direct)
- * or -pos (adjoint) mod 2Ns (SourceCode/TSRK4.c:21-27).  Slot 0 is carried
- * over from the previous chunk's last slot, as TSRK4 carries F3 into k1. */
-static PetscErrorCode stepper_forcing(stepper *st, PetscInt i0, PetscInt steps, int first)
+static void *fslot(stepper *st, int buf, PetscInt j)
+{
+    return (char *)st->fbuf[buf] + (size_t)j * (size_t)st->N * (size_t)st->k * st->esz;
+}
+
+static rsvdt_stream_t frc_stream(void)
+{
+    return g_rsvdt.opt.overlap_forcing ? g_rsvdt.stream_frc : g_rsvdt.stream;
+}
+
+/* Forcing samples of the chunk of steps i0 .. i0+steps-1 (1-based) into
+ * buffer buf: slot j holds half-step position 2(i0-1)+j, j = 0..2 steps,
+ * i.e. iDFT sample +pos (direct) or -pos (adjoint) mod 2Ns
+ * (SourceCode/TSRK4.c:21-27, CreateForcingOnFly.c:17-19). */
+static PetscErrorCode stepper_forcing(stepper *st, int buf, PetscInt i0, PetscInt steps)
 {
     const int64_t L = st->N * st->k;
     const int sgn = st->dir ? 1 : -1;
-    const int oprec = st->prec;
-    if (first) {
-        RSVDT_HIPCHK("CreateForcingOnFly",
-                     rsvdt_hip_coef_gemm(1, st->Nwe, L, st->idft, 1, st->Nwe, 0, 1, st->twoNs, 1.0, st->Fhat, L,
-                                         fslot(st, 0), L, oprec, 0.0, g_rsvdt.stream));
-    } else {
-        RSVDT_HIPCHK("CreateForcingOnFly",
-                     rsvdt_hip_memcpy_d2d(fslot(st, 0), fslot(st, 2 * st->S), (size_t)L * st->esz, g_rsvdt.stream));
-    }
-    const int64_t pos1 = 2 * (i0 - 1) + 1;
+    const int64_t pos0 = 2 * (i0 - 1);
     RSVDT_HIPCHK("CreateForcingOnFly",
-                 rsvdt_hip_coef_gemm(2 * steps, st->Nwe, L, st->idft, 1, st->Nwe, sgn * pos1, sgn, st->twoNs, 1.0,
-                                     st->Fhat, L, fslot(st, 1), L, oprec, 0.0, g_rsvdt.stream));
+                 rsvdt_hip_coef_gemm(2 * steps + 1, st->Nwe, L, st->idft, 1, st->Nwe, sgn * pos0, sgn, st->twoNs, 1.0,
+                                     st->Fhat, L, fslot(st, buf, 0), L, st->prec, 0.0, frc_stream()));
     return 0;
 }
 
 /* One RK4 step (SourceCode/TSRK4.c:18-47) on the whole block, forcing from
- * slots 2s, 2s+1, 2s+2 of the current chunk. */
-static PetscErrorCode stepper_step(stepper *st, PetscInt s, double *snap)
+ * slots 2s, 2s+1, 2s+2 of the chunk's buffer. */
+static PetscErrorCode stepper_step(stepper *st, int buf, PetscInt s, double *snap)
 {
     const double h = st->h;
-    void *F0 = fslot(st, 2 * s), *Fh = fslot(st, 2 * s + 1), *F1 = fslot(st, 2 * s + 2);
+    void *F0 = fslot(st, buf, 2 * s), *Fh = fslot(st, buf, 2 * s + 1), *F1 = fslot(st, buf, 2 * s + 2);
     rsvdt_stream_t q = g_rsvdt.stream;
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 1, st->y, st->y, F0, st->acc, st->u1, h / 2, h / 6, NULL, q));
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 2, st->u1, st->y, Fh, st->acc, st->u2, h / 2, h / 6 * 2, NULL, q));
@@ -136,7 +159,7 @@ static PetscErrorCode stepper_step(stepper *st, PetscInt s, double *snap)
  * elapsed times are read once at the end so that timing never stalls the
  * launch queue. */
 typedef struct { void **ev; int n, cap; double ms; } phase_timer;
-static void pt_begin(phase_timer *t)
+static void pt_begin_on(phase_timer *t, rsvdt_stream_t q)
 {
     if (t->n + 2 > t->cap) {
         int nc = t->cap ? 2 * t->cap : 64;
@@ -147,14 +170,16 @@ static void pt_begin(phase_timer *t)
     }
     if (!t->ev[t->n]) rsvdt_hip_event_create(&t->ev[t->n]);
     if (!t->ev[t->n + 1]) rsvdt_hip_event_create(&t->ev[t->n + 1]);
-    rsvdt_hip_event_record(t->ev[t->n], g_rsvdt.stream);
+    rsvdt_hip_event_record(t->ev[t->n], q);
 }
-static void pt_end(phase_timer *t)
+static void pt_end_on(phase_timer *t, rsvdt_stream_t q)
 {
     if (t->n + 2 > t->cap) return;
-    rsvdt_hip_event_record(t->ev[t->n + 1], g_rsvdt.stream);
+    rsvdt_hip_event_record(t->ev[t->n + 1], q);
     t->n += 2;
 }
+static void pt_begin(phase_timer *t) { pt_begin_on(t, g_rsvdt.stream); }
+static void pt_end(phase_timer *t) { pt_end_on(t, g_rsvdt.stream); }
 static void pt_collect(phase_timer *t)
 {
     double ms = 0;
@@ -173,36 +198,48 @@ static void pt_free(phase_timer *t)
 }
 
 /* ------------------------------------------------------------------ SpMM autotune */
-/* Picks the lanes-per-row of the RK4 stage kernel for this operator and
- * precision by timing five stage-2 launches per candidate; the choice is
- * kept on the Mat and reused by every later action. */
+/* Picks the launch shape of the fused SpMM+stage kernel for this operator,
+ * precision and direction -- lanes per row (the column-block width a row's
+ * lanes cover), gather batch / streaming variant and XCD row remap -- by
+ * timing three stage-2 launches of every candidate on scratch buffers.  The
+ * choice is kept on the Mat and reused by every later action. */
 static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
 {
-    int *slot = &A->tuned_lanes[prec == 32][dir ? 0 : 1];
-    if (*slot || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
-    int cand[6], nc = 0;
-    for (int l = 4; l <= 64; l <<= 1) if (l >= k / 4 && l <= 64 && (k + l - 1) / l <= 4) cand[nc++] = l;
-    if (nc <= 1) { *slot = nc ? cand[0] : 0; return 0; }
+    const int p = prec == 32, q = dir ? 0 : 1;
+    if (A->tuned_ok[p][q] || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
     rsvdt_spmm_desc dsc;
     desc_for(&dsc, A, dir, prec, k, 1);
     void *e0, *e1;
-    rsvdt_hip_event_create(&e0); rsvdt_hip_event_create(&e1);
-    float best = 1e30f; int bestl = 0;
-    for (int i = 0; i < nc; i++) {
-        dsc.lanes = cand[i];
-        /* stage 2 with uin = a, y = b, no forcing, acc = d, out = c: all scratch */
-        int rc = rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
-        if (rc) continue;
-        rsvdt_hip_event_record(e0, g_rsvdt.stream);
-        for (int r = 0; r < 5; r++) rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
-        rsvdt_hip_event_record(e1, g_rsvdt.stream);
-        float ms = 0;
-        rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
-        if (ms < best) { best = ms; bestl = cand[i]; }
+    RSVDT_HIPCHK("autotune", rsvdt_hip_event_create(&e0));
+    RSVDT_HIPCHK("autotune", rsvdt_hip_event_create(&e1));
+    float best = 1e30f;
+    int bl = 0, bv = 3, br = 0;
+    for (int l = 4; l <= 64; l <<= 1) {
+        if ((k + l - 1) / l > 4) continue;            /* at most 4 columns per lane */
+        if (l >= 2 * k && l > 4) continue;            /* no more than half the lanes idle */
+        const int cpl = (int)((k + l - 1) / l);
+        for (int v = 0; v < (cpl == 1 ? 4 : 1); v++)
+            for (int r = 0; r < 2; r++) {
+                dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
+                /* stage 2 with uin = a, y = b, no forcing, acc = d, out = c: all scratch */
+                if (rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream)) continue;
+                rsvdt_hip_event_record(e0, g_rsvdt.stream);
+                for (int rep = 0; rep < 3; rep++) rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
+                rsvdt_hip_event_record(e1, g_rsvdt.stream);
+                float ms = 0;
+                rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+                if (ms < best) { best = ms; bl = l; bv = v; br = r; }
+            }
     }
-    rsvdt_hip_event_destroy(e0); rsvdt_hip_event_destroy(e1);
-    *slot = bestl;
-    if (g_rsvdt.opt.verbose) rsvdt_printf("[rsvdt] autotune: %d lanes per row (%.3f ms per stage)\n", bestl, best / 5);
+    rsvdt_hip_event_destroy(e0);
+    rsvdt_hip_event_destroy(e1);
+    A->tuned_lanes[p][q] = bl;
+    A->tuned_variant[p][q] = bv;
+    A->tuned_remap[p][q] = br;
+    A->tuned_ok[p][q] = 1;
+    if (g_rsvdt.opt.verbose)
+        rsvdt_printf("[rsvdt] autotune (%s, FP%d): %d lanes/row, variant %d, xcd remap %d: %.1f us per stage\n",
+                     dir ? "A" : "A^H", prec, bl, bv, br, 1e3 * best / 3);
     return 0;
 }
 
@@ -229,11 +266,23 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
     }
     int64_t launches = 0;
-    for (PetscInt i0 = 1; i0 <= rend; i0 += st.S) {
-        PetscInt steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
-        if (t_frc) pt_begin(t_frc);
-        RSVDT_CHK(stepper_forcing(&st, i0, steps, i0 == 1));
-        if (t_frc) pt_end(t_frc);
+    const PetscInt nch = (rend + st.S - 1) / st.S;
+    PetscErrorCode ierr = 0;
+    /* forcing of the first two chunks; later chunks are refilled as soon as
+     * the steps that read their buffer are done */
+    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_rk[1], g_rsvdt.stream));   /* zero fill done */
+    for (PetscInt c = 0; c < nch && c < 2; c++) {
+        PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[1]));
+        if (t_frc) pt_begin_on(t_frc, frc_stream());
+        RSVDT_CHK(stepper_forcing(&st, (int)(c & 1), i0, steps));
+        if (t_frc) pt_end_on(t_frc, frc_stream());
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
+    }
+    for (PetscInt c = 0; c < nch && !ierr; c++) {
+        const int buf = (int)(c & 1);
+        const PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
         if (t_rk4) pt_begin(t_rk4);
         for (PetscInt s = 0; s < steps; s++) {
             PetscInt i = i0 + s;
@@ -244,13 +293,31 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
                 if (!dir) pos = Nstore - 1 - pos;
                 snap = Yall->d + 2 * (size_t)pos * (size_t)N * (size_t)k;
             }
-            RSVDT_CHK(stepper_step(&st, s, snap));
+            ierr = stepper_step(&st, buf, s, snap);
+            if (ierr) break;
             launches += 4;
         }
         if (t_rk4) pt_end(t_rk4);
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_rk[buf], g_rsvdt.stream));
+        if (c + 2 < nch && !ierr) {
+            const PetscInt j0 = 1 + (c + 2) * st.S, st2 = st.S < rend - j0 + 1 ? st.S : rend - j0 + 1;
+            RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[buf]));
+            if (t_frc) pt_begin_on(t_frc, frc_stream());
+            ierr = stepper_forcing(&st, buf, j0, st2);
+            if (t_frc) pt_end_on(t_frc, frc_stream());
+            RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[buf], frc_stream()));
+        }
     }
+    /* the main stream must not run ahead of a forcing GEMM still reading F_hat */
+    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_sync());
+    stepper_free(&st);
+    RSVDT_CHK(ierr);
     g_rsvdt.stats.steps = rend;
     g_rsvdt.stats.rk4_launches = launches;
+    g_rsvdt.stats.chunk_steps = (int)st.S;
+    g_rsvdt.stats.spmm_lanes = st.desc.lanes;
+    g_rsvdt.stats.spmm_variant = st.desc.variant;
+    g_rsvdt.stats.spmm_xcd_remap = st.desc.xcd_remap;
     return 0;
 }
 
@@ -293,8 +360,10 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
  * Q is X R^{-1} with R = R3 R2 R1 upper triangular with a positive real
  * diagonal -- the unique factor that SLEPc's Gram-Schmidt BVOrthogonalize
  * also returns for a full-rank input (SourceCode/QRAllFreqs.c:26). */
-PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
-                            double *R_out)
+/* npass CholeskyQR passes in place (the first one shifted when shift_first);
+ * R_out (optional) receives the product of the passes' factors. */
+static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
+                                    int64_t xa, double *R_out, int npass, int shift_first, double drop_tol)
 {
     size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
     double *G = rsvdt_scratch(SCR_G, mat);
@@ -309,9 +378,10 @@ PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, 
     rsvdt_stream_t q = g_rsvdt.stream;
     RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_zero(info, 256, q));
     const double shift_rel = 11.0 * ((double)nrows * (double)n + (double)n * (double)(n + 1)) * 1.1102230246251565e-16;
-    for (int pass = 0; pass < 3; pass++) {
+    for (int pass = 0; pass < npass; pass++) {
         RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, X, xb, xr, xa, X, xb, xr, xa, G, work, q));
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, G, pass == 0 ? shift_rel : 0.0, R, Ri, info, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, G, (pass == 0 && shift_first) ? shift_rel : 0.0, drop_tol,
+                                                       R, Ri, info, q));
         RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, X, xb, xr, xa, Ri, 0.0, NULL, 0, 0, 0, X, xb, xr, xa, q));
         if (R_out) {
             if (pass == 0) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, R, mat, q));
@@ -328,6 +398,49 @@ PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, 
     return 0;
 }
 
+PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
+                            double *R_out, double drop_tol)
+{
+    return cholqr_passes(nb, nrows, n, X, xb, xr, xa, R_out, 3, 1, drop_tol);
+}
+
+PetscErrorCode rsvdt_span_basis(PetscInt nb, PetscInt nrows, PetscInt n, const double *X, double *Q, double *E,
+                                int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol)
+{
+    size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
+    double *C = rsvdt_scratch(SCR_PROJ, mat);
+    double *D = rsvdt_scratch(SCR_MASK, mat);
+    double *Rm = rsvdt_scratch(SCR_RMASK, mat);
+    int *cnt = rsvdt_scratch(SCR_CNT, 256);
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(nb, nrows, n, n));
+    if (!C || !D || !Rm || !cnt || !work) return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate basis work");
+    rsvdt_stream_t q = g_rsvdt.stream;
+    const int64_t nel = (int64_t)(bytes / 16);
+    /* level 1 */
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_memcpy_d2d(Q, X, bytes, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, Rm, 3, 1, drop_tol));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zero(cnt, 256, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bdrop_mask(nb, n, Rm, D, cnt, q));
+    int ndrop = 0;
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_memcpy_d2h(&ndrop, cnt, sizeof(int)));
+    if (ndrop == 0) return 0;
+    /* level 2: E = ((I - Q Q^H) X) D, the residuals of the dropped columns */
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(nb, nrows, n, n, Q, xb, xr, xa, X, xb, xr, xa, C, work, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zscale((int64_t)nb * n * n, -1.0, C, C, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, X, xb, xr, xa, E, xb, xr, xa, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, E, xb, xr, xa, D, 0.0, NULL, 0, 0, 0, E, xb, xr, xa, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, E, xb, xr, xa, NULL, 3, 1, drop_tol));
+    /* reorthogonalise against the level-1 basis and renormalise */
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(nb, nrows, n, n, Q, xb, xr, xa, E, xb, xr, xa, C, work, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zscale((int64_t)nb * n * n, -1.0, C, C, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, E, xb, xr, xa, E, xb, xr, xa, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, E, xb, xr, xa, NULL, 2, 0, drop_tol));
+    /* the level-2 columns take the dropped columns' places; one more pass */
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zaxpy(nel, 1.0, E, Q, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, NULL, 1, 0, drop_tol));
+    return 0;
+}
+
 /* Left singular vectors (descending) of nb tall N x n matrices in place:
  * CholeskyQR, one-sided Jacobi on R, Q <- Q U_R. */
 PetscErrorCode rsvdt_svd_left(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
@@ -337,7 +450,7 @@ PetscErrorCode rsvdt_svd_left(PetscInt nb, PetscInt nrows, PetscInt n, double *X
     double *Rf = rsvdt_scratch(SCR_MISC, mat);
     double *U = rsvdt_scratch(SCR_MISC2, mat);
     if (!Rf || !U) return rsvdt_err(RSVDT_ERR_MEM, "SVDAllFreqs", "cannot allocate SVD work");
-    RSVDT_CHK(rsvdt_cholqr(nb, nrows, n, X, xb, xr, xa, Rf));
+    RSVDT_CHK(rsvdt_cholqr(nb, nrows, n, X, xb, xr, xa, Rf, 0.0));
     RSVDT_HIPCHK("SVDAllFreqs", rsvdt_hip_bjacobi_svd(nb, n, Rf, U, sig_dev, g_rsvdt.stream));
     RSVDT_HIPCHK("SVDAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, X, xb, xr, xa, U, 0.0, NULL, 0, 0, 0, X, xb, xr, xa,
                                                  g_rsvdt.stream));
@@ -399,9 +512,19 @@ static PetscErrorCode snapshot_basis(Mat Yall, PetscInt k, Mat *Vout)
     RSVDT_MatDestroy(&g_trV);
     PetscInt Ns1 = Yall->cols / k, N = Yall->rows;
     if (Yall->order != 1 || Yall->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(Yall, k, 1));
-    RSVDT_CHK(rsvdt_mat_dup(Yall, &g_trV));
-    /* batch = test vector ik: X_ik(r, s) = V[s*N*k + r*k + ik] */
-    RSVDT_CHK(rsvdt_cholqr(k, N, Ns1, g_trV->d, 1, k, N * k, NULL));
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Yall->cols, k, 1, &g_trV));
+    Mat E;
+    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Yall->cols, k, 1, &E));
+    /* batch = test vector ik: X_ik(r, s) = Y_all[s*N*k + r*k + ik].  The
+     * snapshots of one period are a periodic response of rank <= Nw_eff plus
+     * a transient that decays, so the snapshot matrix is often numerically
+     * rank-deficient (singular values down to rounding level); the basis
+     * spans it to rounding accuracy with the numerically dependent
+     * directions left out (zero columns). */
+    PetscErrorCode ierr = rsvdt_span_basis(k, N, Ns1, Yall->d, g_trV->d, E->d, 1, k, N * k,
+                                           (size_t)N * (size_t)Yall->cols * 16, 1e-14);
+    RSVDT_MatDestroy(&E);
+    RSVDT_CHK(ierr);
     g_trV_key = Yall->d;
     *Vout = g_trV;
     return 0;
@@ -446,7 +569,15 @@ PetscErrorCode QRDecomposition(Mat Vm)
 {
     if (Vm->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "QRDecomposition", "dense Mat required");
     if (Vm->kb != Vm->cols) RSVDT_CHK(rsvdt_mat_to_layout(Vm, Vm->cols, 0));
-    return rsvdt_cholqr(1, Vm->rows, Vm->cols, Vm->d, 0, Vm->cols, 1, NULL);
+    Mat X = NULL, E = NULL;
+    RSVDT_CHK(rsvdt_mat_dup(Vm, &X));
+    PetscErrorCode ierr = rsvdt_mat_alloc_dense(Vm->rows, Vm->cols, Vm->cols, 0, &E);
+    if (!ierr)
+        ierr = rsvdt_span_basis(1, Vm->rows, Vm->cols, X->d, Vm->d, E->d, 0, Vm->cols, 1,
+                                (size_t)Vm->rows * (size_t)Vm->cols * 16, 1e-14);
+    RSVDT_MatDestroy(&X);
+    RSVDT_MatDestroy(&E);
+    return ierr;
 }
 
 /* SourceCode/GalerkinProjAllModes.c:28-42 with GalerkinProj.c:26-74, for all

@@ -35,7 +35,9 @@ struct _p_RSVDT_Mat {
     double *h_val;           /* 2*nnz */
     int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
     void *d_val64, *d_val32, *d_valH64, *d_valH32;
-    int tuned_lanes[2][2];   /* autotuned lanes per row: [precision 64/32][direct/adjoint] */
+    /* autotuned SpMM launch shape per [precision 64/32][direct/adjoint]:
+     * lanes per row, kernel variant, XCD remap; tuned_ok marks a tuned entry */
+    int tuned_lanes[2][2], tuned_variant[2][2], tuned_remap[2][2], tuned_ok[2][2];
 };
 
 struct _p_RSVDT_Vec {
@@ -47,7 +49,8 @@ struct _p_RSVDT_Vec {
 typedef struct {
     int initialized;
     int device;
-    rsvdt_stream_t stream;
+    rsvdt_stream_t stream;      /* all work of a run */
+    rsvdt_stream_t stream_frc;  /* forcing-chunk GEMMs, overlapped with the RK4 steps */
     RSVDT_Options opt;
     RSVDT_ActionStats stats;
     /* learned-sketch columns for CreateRandomMat */
@@ -94,15 +97,26 @@ PetscErrorCode rsvdt_aij_real_check(Mat m);
 PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps, double *X, PetscInt N,
                                        PetscInt k, PetscInt nb);
 /* Shifted CholeskyQR3 of nbatch tall N x n matrices given by strides, in
- * place; R (nbatch x n x n, device) optional. */
+ * place; R (nbatch x n x n, device) optional.  drop_tol > 0 (bases that may
+ * be rank-deficient): columns whose second/third-pass pivot falls below
+ * drop_tol times the largest Gram diagonal are zeroed instead of normalized. */
 PetscErrorCode rsvdt_cholqr(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
-                            int64_t xa, double *R_out);
+                            int64_t xa, double *R_out, double drop_tol);
+/* Orthonormal basis Q of the column span of nbatch tall matrices X (same
+ * strides; Q and the work array E each cover `bytes` bytes like X) that may
+ * be numerically rank-deficient: CholeskyQR3 with dropping, then the same on
+ * the residuals (I - Q Q^H) X of the dropped columns, reorthogonalised, in
+ * the dropped columns' places.  Columns dependent to rounding stay zero. */
+PetscErrorCode rsvdt_span_basis(PetscInt nbatch, PetscInt nrows, PetscInt n, const double *X, double *Q, double *E,
+                                int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol);
 /* Left singular vectors (descending) of nbatch tall matrices, in place, and
  * their singular values (device, nbatch x n). */
 PetscErrorCode rsvdt_svd_left(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
                               int64_t xa, double *sig_dev);
 
-/* device scratch that is reused across calls (grown on demand, freed at Finalize) */
+/* device scratch that is reused across calls (grown on demand, freed at
+ * Finalize), in RSVDT_NSCRATCH numbered slots */
+#define RSVDT_NSCRATCH 32
 void *rsvdt_scratch(int slot, size_t bytes);
 void rsvdt_scratch_free_all(void);
 

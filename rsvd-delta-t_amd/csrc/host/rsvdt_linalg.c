@@ -27,7 +27,7 @@ PetscErrorCode QRAllFreqs(RSVD_matrices *R, RSVDt_vars *V)
     RSVDT_CHK(freq_blocks(R->Y_hat, k, "QRAllFreqs"));
     Mat Y = R->Y_hat;
     PetscInt nb = Y->cols / k, N = Y->rows;
-    RSVDT_CHK(rsvdt_cholqr(nb, N, k, Y->d, N * k, k, 1, NULL));
+    RSVDT_CHK(rsvdt_cholqr(nb, N, k, Y->d, N * k, k, 1, NULL, 0.0));
     RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_sync());
     if (V->Display > 0) rsvdt_printf("*** QR decomposition elapsed time = %.3f s ***\n", RSVDT_Time() - t1);
     return 0;

@@ -196,8 +196,9 @@ static void default_options(RSVDT_Options *o)
     o->spmm_lanes = 0;
     o->xcd_remap = 1;
     o->use_graph = 0;
-    o->autotune = 0;
+    o->autotune = 1;
     o->verbose = 0;
+    o->overlap_forcing = 1;
 }
 
 PetscErrorCode RSVDT_Initialize(int *argc, char ***argv)
@@ -234,13 +235,20 @@ PetscErrorCode rsvdt_ensure_device(void)
     int rc = rsvdt_hip_init(dev);
     if (rc) return rsvdt_err(rc, "RSVDT_Initialize", "%s", rsvdt_hip_last_error());
     g_rsvdt.device = dev;
-    g_rsvdt.stream = NULL;   /* the null stream: every kernel of a run is ordered on it */
+    rc = rsvdt_hip_stream_create(&g_rsvdt.stream);
+    if (!rc) rc = rsvdt_hip_stream_create(&g_rsvdt.stream_frc);
+    if (rc) return rsvdt_err(rc, "RSVDT_Initialize", "%s", rsvdt_hip_last_error());
     return 0;
 }
 
 PetscErrorCode RSVDT_Finalize(void)
 {
     rsvdt_scratch_free_all();
+    if (g_rsvdt.device >= 0) {
+        rsvdt_hip_stream_destroy(g_rsvdt.stream);
+        rsvdt_hip_stream_destroy(g_rsvdt.stream_frc);
+        g_rsvdt.stream = g_rsvdt.stream_frc = NULL;
+    }
     free(g_rsvdt.sketch_host);
     g_rsvdt.sketch_host = NULL;
     RSVDT_OptionsClear();
@@ -286,7 +294,7 @@ PetscErrorCode RSVDT_SetSketchColumns(PetscInt N, PetscInt ncols, PetscInt Nw_ef
 }
 
 /* ------------------------------------------------------------------ scratch */
-#define NSCRATCH 16
+#define NSCRATCH RSVDT_NSCRATCH
 static void *g_scr[NSCRATCH];
 static size_t g_scr_sz[NSCRATCH];
 
@@ -531,7 +539,7 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     PetscErrorCode ierr = upload_csr(nr, rp, ci, v, nnz, m->is_real, &m->d_rowptrH, &m->d_colH, &m->d_valH64,
                                      &m->d_valH32);
     free(rp); free(ci); free(v); free(fill);
-    m->tuned_lanes[0][0] = m->tuned_lanes[0][1] = m->tuned_lanes[1][0] = m->tuned_lanes[1][1] = 0;
+    memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
     return ierr;
 }
 

@@ -94,3 +94,26 @@ def test_fp32_state_form(pkg):
     pkg.set_options(precision=64)
     print(f"fp32 relative error {e:.2e}")
     assert e < 2e-5
+
+
+@pytest.mark.parametrize("TL", [5.0, 20.0])
+def test_rank_deficient_snapshot_basis(pkg, TL):
+    """Real operator, Nw_eff = 16 < Nw + 1 = 33 snapshots and a transient that
+    has decayed: the snapshot basis is numerically rank-deficient (singular
+    values down to 1e-17 of the largest).  The removal must still return the
+    periodic response (exact discrete answer) and agree with the oracle."""
+    print(f"rank-deficient snapshots TL={TL}", flush=True)
+    A = kat.gl_operator(N=120, L=30.0).real.astype(complex)
+    rng = np.random.default_rng(int(TL))
+    o = oracle.Oracle(A, k=2, q=0, Nw=32, w=0.3, TL=TL, dt=0.02, transient_removal=True)
+    F = crandn(rng, 120, 2 * o.Nw_eff)
+    Yo = o.action(F)
+    Yp, _ = kat.exact_action(A, F, 2, 32, 0.3, TL, 0.02, transient_removal=True)
+    s = pkg.Session(A, k=2, q=0, Nw=32, w=0.3, TL=TL, dt=0.02, transient_removal=True)
+    s.set_sketch(F)
+    s.direct()
+    Yg = s.get_Y()
+    s.close()
+    assert np.all(np.isfinite(Yg))
+    assert rel(Yg, Yp) < 1e-9
+    assert rel(Yg, Yo) < 1e-9

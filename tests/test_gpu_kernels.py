@@ -85,8 +85,8 @@ def test_chol_jacobi_solve(L, pkg, nb, n):
     dG = Dev(pkg, G)
     dR, dRi = Dev(pkg, nbytes=G.nbytes), Dev(pkg, nbytes=G.nbytes)
     info = Dev(pkg, np.zeros(4, dtype=np.int32))
-    L.rsvdt_hip_bchol_inv.argtypes = [I64, I64, V, ctypes.c_double, V, V, V, V]
-    assert L.rsvdt_hip_bchol_inv(nb, n, dG.p, 0.0, dR.p, dRi.p, info.p, None) == 0
+    L.rsvdt_hip_bchol_inv.argtypes = [I64, I64, V, ctypes.c_double, ctypes.c_double, V, V, V, V]
+    assert L.rsvdt_hip_bchol_inv(nb, n, dG.p, 0.0, 0.0, dR.p, dRi.p, info.p, None) == 0
     R = dR.get(np.complex128, G.shape)
     Ri = dRi.get(np.complex128, G.shape)
     for b in range(nb):
@@ -135,15 +135,17 @@ def rk4_ref(A, y, F0, Fh, F1, h):
 
 class Desc(ctypes.Structure):
     _fields_ = [("nrows", I64), ("k", I64), ("nb", I64), ("rowptr", V), ("col", V), ("val", V), ("a_real", ctypes.c_int),
-                ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int)]
+                ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
-@pytest.mark.parametrize("k,real,prec,nb,lanes", [(1, False, 64, 1, 0), (3, True, 64, 1, 0), (10, False, 64, 1, 0),
-                                                  (16, True, 64, 1, 0), (16, False, 64, 1, 8), (32, False, 64, 3, 0),
-                                                  (40, True, 64, 1, 0), (100, False, 64, 1, 0), (16, True, 32, 1, 0),
-                                                  (10, False, 32, 2, 4)])
-def test_rk4_step(L, pkg, k, real, prec, nb, lanes):
-    print(f"rk4 k={k} real={real} prec={prec} nb={nb} lanes={lanes}", flush=True)
+@pytest.mark.parametrize("k,real,prec,nb,lanes,variant", [
+    (1, False, 64, 1, 0, 0), (3, True, 64, 1, 0, 0), (10, False, 64, 1, 0, 0), (16, True, 64, 1, 0, 0),
+    (16, False, 64, 1, 8, 0), (32, False, 64, 3, 0, 0), (40, True, 64, 1, 0, 0), (100, False, 64, 1, 0, 0),
+    (16, True, 32, 1, 0, 0), (10, False, 32, 2, 4, 0),
+    (16, True, 64, 2, 16, 1), (16, True, 64, 1, 16, 2), (16, False, 64, 1, 16, 3), (16, True, 32, 1, 16, 3),
+    (8, True, 64, 2, 32, 3)])
+def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
+    print(f"rk4 k={k} real={real} prec={prec} nb={nb} lanes={lanes} variant={variant}", flush=True)
     rng = np.random.default_rng(k + nb)
     N = 3001
     A = sp.random(N, N, density=5.0 / N, random_state=k, format="csr", dtype=np.float64)
@@ -161,7 +163,7 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes):
     dy = Dev(pkg, y.astype(cdt)); dF = [Dev(pkg, F[i].astype(cdt)) for i in range(3)]
     du1, du2, dacc = (Dev(pkg, nbytes=y.astype(cdt).nbytes) for _ in range(3))
     dsnap = Dev(pkg, nbytes=y.nbytes)
-    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, lanes, 1)
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, lanes, 1, variant)
     L.rsvdt_hip_rk4_stage.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, V, V, V, V, V, ctypes.c_double,
                                       ctypes.c_double, V, V]
     f = L.rsvdt_hip_rk4_stage

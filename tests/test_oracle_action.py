@@ -88,3 +88,20 @@ def test_snapshot_store_layout():
     o.set_max_steps(o.Nt + o.RR)
     Yq = o.snapshots(F, direct=False)
     assert np.allclose(Yq[:, 4], Ya[:, 4], rtol=0, atol=1e-14 * np.abs(Ya).max())
+
+
+@pytest.mark.parametrize("TL", [5.0, 20.0, 60.0])
+def test_rank_deficient_snapshots(TL):
+    """Snapshots of a real operator with Nw_eff = 16 frequencies and a decayed
+    transient are numerically rank-deficient (17 of 33 singular values below
+    1e-10 of the largest); the Householder basis completes them with
+    rounding-noise directions and the removal still returns the periodic part."""
+    A = kat.gl_operator(N=120, L=30.0).real.astype(complex)
+    rng = np.random.default_rng(0)
+    o = oracle.Oracle(A, k=2, q=0, Nw=32, w=0.3, TL=TL, dt=0.02, transient_removal=True)
+    F = crandn(rng, 120, 2 * o.Nw_eff)
+    Ys = o.snapshots(F)
+    sv = np.linalg.svd(Ys[:, :33], compute_uv=False)
+    assert sv[-1] / sv[0] < 1e-15
+    Yp, _ = kat.exact_action(A, F, 2, 32, 0.3, TL, 0.02, transient_removal=True)
+    assert rel(o.action(F), Yp) < 1e-9
