@@ -60,6 +60,31 @@ def cpu_baseline(A, k, Nw, w, TL, dt, bytes_per_step, budget_s=12.0):
                       f"scaled to k={k} vectors per block step"}
 
 
+def calib_copy(pkg, gib=1, reps=5):
+    """A plain device copy of known size (gib GiB read + gib GiB written per
+    launch, reps launches of rsvdt_hip_zcopy3): the reference reading for
+    scaling the profiler's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    L = pkg.lib()
+    if L.rsvdt_hip_init(0) != 0:
+        raise RuntimeError("no HIP device")
+    n = (gib << 30) // 16
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if L.rsvdt_hip_malloc(ctypes.byref(p), ctypes.c_size_t(n * 16)) != 0:
+            raise RuntimeError("device allocation failed")
+    L.rsvdt_hip_zero(a, ctypes.c_size_t(n * 16), None)
+    for _ in range(reps):
+        if L.rsvdt_hip_zcopy3(ctypes.c_int64(1), ctypes.c_int64(n), ctypes.c_int64(1), a, ctypes.c_int64(0),
+                              ctypes.c_int64(1), ctypes.c_int64(0), b, ctypes.c_int64(0), ctypes.c_int64(1),
+                              ctypes.c_int64(0), None) != 0:
+            raise RuntimeError("copy failed")
+    L.rsvdt_hip_sync()
+    L.rsvdt_hip_free(a); L.rsvdt_hip_free(b)
+    print(json.dumps({"calib_copy": {"bytes_read_per_launch": n * 16, "bytes_written_per_launch": n * 16,
+                                     "launches": reps, "kernel": "k_zcopy3"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,14 +95,19 @@ def main():
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-overlap", action="store_true", help="forcing GEMM on the RK4 stream")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB device copy 5 times (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards test vectors "
               "and frequencies and is not built", file=sys.stderr)
         sys.exit(2)
 
-    import workloads
     pkg = load_pkg()
+    if args.calib_copy:
+        calib_copy(pkg)
+        return
+    import workloads
     A = workloads.flow2d()
     N = A.shape[0]
     k, Nw, w = 16, 128, 2 * np.pi / 20

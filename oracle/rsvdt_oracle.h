@@ -18,7 +18,8 @@
  *   orc_action (no TR)      exact discrete solution incl. transient, dense
  *                           linear algebra in numpy (test_oracle_action)
  *   orc_action (TR)         exactness when Nw+1 = N (projection onto the full
- *                           space), and transient-error reduction (test_oracle_tr)
+ *                           space), transient-error reduction, rank-deficient
+ *                           snapshot sets (test_oracle_action)
  *   orc_qr                  numpy/LAPACK QR with positive diagonal (test_oracle_linalg)
  *   orc_svd_left            numpy/LAPACK SVD, phase-invariant (test_oracle_linalg)
  *   orc_lu_solve            numpy.linalg.solve (test_oracle_linalg)
