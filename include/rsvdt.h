@@ -220,6 +220,8 @@ typedef struct {
     int autotune;         /* time SpMM lane widths once per operator and keep the fastest */
     int verbose;          /* print phase timings */
     int overlap_forcing;  /* compute the next forcing chunk on a second stream during the RK4 steps */
+    int rk4_form;         /* 0: RK4 as y' = P(hA) y + g (Horner stages, forcing folded into one
+                             precomputed term per step); 1: the reference's stage order */
 } RSVDT_Options;
 PetscErrorCode RSVDT_GetOptions(RSVDT_Options *o);
 PetscErrorCode RSVDT_SetOptions(const RSVDT_Options *o);
@@ -235,6 +237,7 @@ typedef struct {
     int64_t rk4_launches;  /* RK4 stage launches */
     int chunk_steps;       /* RK4 steps per forcing chunk */
     int spmm_lanes, spmm_variant, spmm_xcd_remap;   /* SpMM launch shape used */
+    int rk4_form;          /* stepping form used (RSVDT_Options.rk4_form) */
 } RSVDT_ActionStats;
 PetscErrorCode RSVDT_GetActionStats(RSVDT_ActionStats *s);
 

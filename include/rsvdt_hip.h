@@ -92,6 +92,19 @@ int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,
                         void *acc, void *out, double c_next, double c_acc,
                         double *snap, rsvdt_stream_t s);
 
+/* One Horner stage of the RK4 step of a linear system written as
+ *   y_{i+1} = P(hA) y_i + g_i,  P(z) = 1 + z(1 + z/2(1 + z/3(1 + z/4))):
+ *   v1 = y + (h/4) A y;  v2 = y + (h/3) A v1;  v3 = y + (h/2) A v2;
+ *   y' = y + h A v3 + g_i.
+ * out_b = ys_b * y_b + c * (A vin_b) (+ g_b) for every block b < d->nb
+ * (blocks N x k row-major at stride N*k; the same N x k block layout for
+ * vin, y, g, out and snap).  yscale: nb complex FP64 block scales, NULL = 1.
+ * g: NULL = no forcing term.  out may alias y (never vin).  snap (complex
+ * FP64, may be NULL) receives a copy of out.  d->variant and d->lanes pick
+ * the launch shape as for rsvdt_hip_rk4_stage. */
+int rsvdt_hip_horner_stage(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
+                           const void *g, double c, void *out, double *snap, rsvdt_stream_t s);
+
 /* Plain block SpMM: out_b = M * in_b for nb blocks of k columns (complex FP64
  * values, int32 CSR).  in blocks are ncols x k (stride in_bstride), out blocks
  * nrows x k (stride out_bstride).  Used for the B, C and weight matrices.   */

@@ -90,13 +90,13 @@ class Directories(ctypes.Structure):
 
 class RSVDT_Options(ctypes.Structure):
     _fields_ = [("precision", _int), ("chunk_steps", _int), ("spmm_lanes", _int), ("xcd_remap", _int),
-                ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int)]
+                ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int), ("rk4_form", _int)]
 
 
 class RSVDT_ActionStats(ctypes.Structure):
     _fields_ = [("rk4_ms", _dbl), ("forcing_ms", _dbl), ("dft_ms", _dbl), ("tr_ms", _dbl), ("total_ms", _dbl),
                 ("steps", _i64), ("rk4_launches", _i64), ("chunk_steps", _int), ("spmm_lanes", _int),
-                ("spmm_variant", _int), ("spmm_xcd_remap", _int)]
+                ("spmm_variant", _int), ("spmm_xcd_remap", _int), ("rk4_form", _int)]
 
 
 # Every symbol the C headers (include/rsvdt.h, include/rsvdt_hip.h) declare.
@@ -116,7 +116,7 @@ HIP_ENTRY_POINTS = [
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
-    "rsvdt_hip_rk4_stage", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
+    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
     "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
     "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2",
@@ -257,9 +257,16 @@ def get_options():
     return o
 
 
+# engine option defaults (RSVDT_Initialize's); a Session starts from these
+OPTION_DEFAULTS = dict(precision=64, chunk_steps=0, spmm_lanes=0, xcd_remap=1, use_graph=0, autotune=1, verbose=0,
+                       overlap_forcing=1, rk4_form=0)
+
+
 def set_options(**kw):
     o = get_options()
     for k, v in kw.items():
+        if k not in OPTION_DEFAULTS:
+            raise KeyError(f"unknown engine option {k!r}")
         setattr(o, k, int(v))
     check(lib().RSVDT_SetOptions(ctypes.byref(o)), "RSVDT_SetOptions")
 
@@ -276,7 +283,7 @@ class Session:
     def __init__(self, A, k, q, Nw, w, TL, dt, transient_removal, display=0, precision=64, **opts):
         L = lib()
         check(L.RSVDT_Initialize(None, None), "RSVDT_Initialize")
-        set_options(precision=precision, **opts)
+        set_options(**dict(OPTION_DEFAULTS, precision=precision, **opts))
         import scipy.sparse as sp
         A = sp.csr_matrix(A)
         self.V = RSVDt_vars()

@@ -57,51 +57,18 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 v, bool nt) {
     else *p = v;
 }
 
-// NB: nonzeros gathered per batch (16, or 8 for fewer registers and more
-// waves); NT: nontemporal hints on the row-local streams, which keeps the
-// L2 for the gathered stage vector.
-template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
-__global__ __launch_bounds__(256) void k_rk4_stage(
-    int nrows, int k, int stage, int xcd_remap, int64_t bstride,
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
-    const typename vec2<T>::type *__restrict__ uin, const typename vec2<T>::type *yv,
-    const typename vec2<T>::type *__restrict__ F, typename vec2<T>::type *__restrict__ acc,
-    typename vec2<T>::type *out, T cnext, T cacc, double2 *__restrict__ snap)
+// s[t] += sum over the row's nonzeros e of A(row, col_e) * x(col_e, lane + t*LPR):
+// the row's (column, value) pairs are fetched NB at a time with one coalesced
+// load per lane and broadcast by shuffles, then the NB gathered rows of x are
+// loaded back to back before any FMA.
+template <typename T, bool AREAL, int LPR, int CPL, int NB>
+__device__ __forceinline__ void row_gather(int e0, int e1, int lane, int k, const int32_t *__restrict__ colidx,
+                                           const typename aval<T, AREAL>::type *__restrict__ vals,
+                                           const typename vec2<T>::type *__restrict__ ub, T (&sr)[CPL], T (&si)[CPL])
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
-    constexpr int RPB = 256 / LPR;                  // rows per workgroup
     constexpr int EPL = LPR >= NB ? 1 : NB / LPR;   // entries each lane fetches per batch
-    int bx = blockIdx.x;
-    if (xcd_remap) {
-        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
-        bx = x * q + (x < r ? x : r) + idx;
-    }
-    const int lane = threadIdx.x & (LPR - 1);
-    const int row = bx * RPB + (int)(threadIdx.x / LPR);
-    if (row >= nrows) return;
-    const int64_t boff = (int64_t)blockIdx.y * bstride;
-    const V *ub = uin + boff;
-    const AV *vals = (const AV *)vals_;
-    const int e0 = rowptr[row], e1 = rowptr[row + 1];
-
-    // row-local operands first
-    V fr[CPL], yr[CPL], ar[CPL];
-#pragma unroll
-    for (int t = 0; t < CPL; ++t) {
-        const int j = lane + t * LPR;
-        const int64_t idx = boff + (int64_t)row * k + j;
-        fr[t].x = 0; fr[t].y = 0; yr[t] = fr[t]; ar[t] = fr[t];
-        if (j < k) {
-            if (F != nullptr) fr[t] = ld_stream(F + idx, NT);
-            if (stage != 4) yr[t] = ld_stream(yv + idx, NT);
-            if (stage != 1) ar[t] = ld_stream(acc + idx, NT);
-        }
-    }
-    T sr[CPL], si[CPL];
-#pragma unroll
-    for (int t = 0; t < CPL; ++t) { sr[t] = fr[t].x; si[t] = fr[t].y; }
-
     for (int eb = e0; eb < e1; eb += NB) {
         int mc[EPL];
         AV mv[EPL];
@@ -146,6 +113,53 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
             }
         }
     }
+}
+
+// NB: nonzeros gathered per batch (16, or 8 for fewer registers and more
+// waves); NT: nontemporal hints on the row-local streams, which keeps the
+// L2 for the gathered stage vector.
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
+__global__ __launch_bounds__(256) void k_rk4_stage(
+    int nrows, int k, int stage, int xcd_remap, int64_t bstride,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
+    const typename vec2<T>::type *__restrict__ uin, const typename vec2<T>::type *yv,
+    const typename vec2<T>::type *__restrict__ F, typename vec2<T>::type *__restrict__ acc,
+    typename vec2<T>::type *out, T cnext, T cacc, double2 *__restrict__ snap)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    constexpr int RPB = 256 / LPR;                  // rows per workgroup
+    int bx = blockIdx.x;
+    if (xcd_remap) {
+        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
+        bx = x * q + (x < r ? x : r) + idx;
+    }
+    const int lane = threadIdx.x & (LPR - 1);
+    const int row = bx * RPB + (int)(threadIdx.x / LPR);
+    if (row >= nrows) return;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const V *ub = uin + boff;
+    const AV *vals = (const AV *)vals_;
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+
+    // row-local operands first
+    V fr[CPL], yr[CPL], ar[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + t * LPR;
+        const int64_t idx = boff + (int64_t)row * k + j;
+        fr[t].x = 0; fr[t].y = 0; yr[t] = fr[t]; ar[t] = fr[t];
+        if (j < k) {
+            if (F != nullptr) fr[t] = ld_stream(F + idx, NT);
+            if (stage != 4) yr[t] = ld_stream(yv + idx, NT);
+            if (stage != 1) ar[t] = ld_stream(acc + idx, NT);
+        }
+    }
+    T sr[CPL], si[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) { sr[t] = fr[t].x; si[t] = fr[t].y; }
+
+    row_gather<T, AREAL, LPR, CPL, NB>(e0, e1, lane, k, colidx, vals, ub, sr, si);
 #pragma unroll
     for (int t = 0; t < CPL; ++t) {
         const int j = lane + t * LPR;
@@ -219,6 +233,122 @@ static int dispatch_lanes(const rsvdt_spmm_desc *d, int stage, const void *uin, 
     return RSVDT_HIP_ERR_ARG;
 }
 
+// ---------------------------------------------------------------------------
+// Horner stage.  For a linear operator the classical RK4 step is the degree-4
+// Taylor polynomial of hA applied to y plus a forcing term that is linear in
+// the three stage forcings:
+//   y_{i+1} = P(hA) y_i + g_i,  P(z) = 1 + z(1 + z/2(1 + z/3(1 + z/4))),
+// evaluated as four SpMM stages that each read y and one gathered vector:
+//   v1 = y + (h/4) A y ;  v2 = y + (h/3) A v1 ;  v3 = y + (h/2) A v2 ;
+//   y' = y + h A v3 + g_i
+// One stage: out_b = ys_b * y_b + c * (A vin_b) (+ g_b), ys_b a complex scale
+// per block (1 when ys is null; the forcing precompute uses it), optional
+// FP64 copy of out into snap.
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
+__global__ __launch_bounds__(256) void k_horner_stage(
+    int nrows, int k, int xcd_remap, int64_t bstride,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
+    const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
+    const double2 *__restrict__ ys, const typename vec2<T>::type *__restrict__ g,
+    typename vec2<T>::type *out, T c, double2 *__restrict__ snap)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    constexpr int RPB = 256 / LPR;
+    int bx = blockIdx.x;
+    if (xcd_remap) {
+        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
+        bx = x * q + (x < r ? x : r) + idx;
+    }
+    const int lane = threadIdx.x & (LPR - 1);
+    const int row = bx * RPB + (int)(threadIdx.x / LPR);
+    if (row >= nrows) return;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const AV *vals = (const AV *)vals_;
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+
+    V yr[CPL], gr[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + t * LPR;
+        const int64_t idx = boff + (int64_t)row * k + j;
+        yr[t].x = 0; yr[t].y = 0; gr[t] = yr[t];
+        if (j < k) {
+            yr[t] = ld_stream(yv + idx, NT);
+            if (g != nullptr) gr[t] = ld_stream(g + idx, NT);
+        }
+    }
+    if (ys != nullptr) {
+        const double2 sc = ys[blockIdx.y];
+        const T cr = (T)sc.x, ci = (T)sc.y;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const V v = yr[t];
+            yr[t].x = cr * v.x - ci * v.y;
+            yr[t].y = cr * v.y + ci * v.x;
+        }
+    }
+    T sr[CPL], si[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) { sr[t] = 0; si[t] = 0; }
+    row_gather<T, AREAL, LPR, CPL, NB>(e0, e1, lane, k, colidx, vals, vin + boff, sr, si);
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + t * LPR;
+        if (j >= k) continue;
+        const int64_t idx = boff + (int64_t)row * k + j;
+        V o;
+        o.x = yr[t].x + c * sr[t];
+        o.y = yr[t].y + c * si[t];
+        if (g != nullptr) { o.x += gr[t].x; o.y += gr[t].y; }
+        out[idx] = o;                         // gathered by the next stage: keep it cacheable
+        if (snap != nullptr) st_stream(snap + idx, make_double2((double)o.x, (double)o.y), NT);
+    }
+}
+
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
+static int launch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
+                         double c, void *out, double *snap, hipStream_t s)
+{
+    using V = typename vec2<T>::type;
+    constexpr int RPB = 256 / LPR;
+    const unsigned gx = ceil_div(d->nrows, RPB);
+    const unsigned gy = (unsigned)d->nb;
+    const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
+    hipLaunchKernelGGL((k_horner_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows,
+                       (int)d->k, remap, d->nrows * d->k, d->rowptr, d->col, d->val, (const V *)vin, (const V *)y,
+                       (const double2 *)ys, (const V *)g, (V *)out, (T)c, (double2 *)snap);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
+    return RSVDT_HIP_OK;
+}
+
+template <typename T, bool AREAL>
+static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
+                           double c, void *out, double *snap, hipStream_t s)
+{
+    int lpr = d->lanes;
+    const int k = (int)d->k;
+    if (lpr == 0) {
+        lpr = 4;
+        while (lpr < k && lpr < 64) lpr <<= 1;
+    }
+    const int cpl = (k + lpr - 1) / lpr;
+    const int var = d->variant;
+#define RSVDT_HCASE(L, C)                                                                                         \
+    if (lpr == L && cpl <= C) {                                                                                   \
+        if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, s); \
+        if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, s); \
+        if (C == 1 && var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, s);  \
+        return launch_horner<T, AREAL, L, C, 16, false>(d, vin, y, ys, g, c, out, snap, s);                        \
+    }
+    RSVDT_HCASE(4, 1) RSVDT_HCASE(8, 1) RSVDT_HCASE(16, 1) RSVDT_HCASE(32, 1) RSVDT_HCASE(64, 1)
+    RSVDT_HCASE(4, 2) RSVDT_HCASE(8, 2) RSVDT_HCASE(16, 2) RSVDT_HCASE(32, 2) RSVDT_HCASE(64, 2)
+    RSVDT_HCASE(4, 4) RSVDT_HCASE(8, 4) RSVDT_HCASE(16, 4) RSVDT_HCASE(32, 4) RSVDT_HCASE(64, 4)
+#undef RSVDT_HCASE
+    set_error_msg("rsvdt_hip_horner_stage: unsupported k / lanes combination (k <= 256)");
+    return RSVDT_HIP_ERR_ARG;
+}
+
 }  // namespace rsvdt
 
 using namespace rsvdt;
@@ -249,6 +379,40 @@ extern "C" int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage, const vo
         return dispatch_lanes<float, false>(d, stage, uin, y, F, acc, out, c_next, c_acc, snap, st);
     }
     set_error_msg("rsvdt_hip_rk4_stage: prec must be 64 or 32");
+    return RSVDT_HIP_ERR_ARG;
+}
+
+static int check_desc(const rsvdt_spmm_desc *d, const char *who)
+{
+    if (!d || d->nrows <= 0 || d->k <= 0 || d->nb <= 0 || d->nrows > 0x7fffffff || d->nb > 65535 ||
+        (d->lanes && (d->lanes & (d->lanes - 1)))) {
+        char msg[160];
+        snprintf(msg, sizeof(msg), "%s: bad descriptor (nrows, k, nb, lanes)", who);
+        set_error_msg(msg);
+        return RSVDT_HIP_ERR_ARG;
+    }
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_horner_stage(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
+                                      const void *g, double c, void *out, double *snap, rsvdt_stream_t s)
+{
+    int rc = check_desc(d, "rsvdt_hip_horner_stage");
+    if (rc) return rc;
+    if (!vin || !y || !out) {
+        set_error_msg("rsvdt_hip_horner_stage: vin, y and out are required");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    hipStream_t st = as_stream(s);
+    if (d->prec == 64) {
+        if (d->a_real) return dispatch_horner<double, true>(d, vin, y, yscale, g, c, out, snap, st);
+        return dispatch_horner<double, false>(d, vin, y, yscale, g, c, out, snap, st);
+    }
+    if (d->prec == 32) {
+        if (d->a_real) return dispatch_horner<float, true>(d, vin, y, yscale, g, c, out, snap, st);
+        return dispatch_horner<float, false>(d, vin, y, yscale, g, c, out, snap, st);
+    }
+    set_error_msg("rsvdt_hip_horner_stage: prec must be 64 or 32");
     return RSVDT_HIP_ERR_ARG;
 }
 

@@ -18,23 +18,44 @@
 
 /* scratch slots */
 enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_FBUF2, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP,
-       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_NSLOTS };
+       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_FSCALE, SCR_W1, SCR_W2, SCR_NSLOTS };
 _Static_assert(SCR_NSLOTS <= RSVDT_NSCRATCH, "scratch slot table too small");
 
 /* ------------------------------------------------------------------ stepper */
-/* Forcing chunks are double-buffered: while the RK4 steps of chunk c run on
- * the main stream from buffer c%2, the forcing GEMM of chunk c+1 (MFMA-bound)
- * fills the other buffer on the forcing stream, ordered by events. */
+/* Two forms of the RK4 step of SourceCode/TSRK4.c:18-47 on the whole N x k
+ * block of test vectors.
+ *
+ * form 0 (default).  For the linear system y' = A y + f(t) the step is
+ *   y_{i+1} = P(hA) y_i + g_i,   P(z) = 1 + z + z^2/2 + z^3/6 + z^4/24,
+ * and, since the stage forcings of step i are the iDFT samples at half-steps
+ * 2(i-1), 2i-1, 2i (TSRK4.c:21-27), i.e. (1, z_m, z_m^2) z_m^{2(i-1)} F_hat_m/Ns
+ * per frequency m with z_m the half-step phase, the forcing term is
+ *   g_i = sum_m idft(m, 2(i-1)) G_m,   G_m = Q(z_m) F_hat_m,
+ *   Q(z) = h/6 [(1+4z+z^2) + (1+2z) hA + (1+z)/2 (hA)^2 + (hA)^3/4],
+ * the forced RK4 step from y = 0 written out.  G is computed once per action
+ * (3 SpMMs per frequency block) in place of F_hat, a forcing chunk holds one
+ * sample per step, and P(hA) y is evaluated by four Horner stages that each
+ * read y and one gathered vector (12 block reads/writes per step instead of
+ * the 20 of the stage form).  Same polynomial, same forcing, different
+ * rounding order.
+ *
+ * form 1: the reference's stage order (k_rk4_stage), forcing chunks of 2S+1
+ * half-step samples.
+ *
+ * Forcing chunks are double-buffered: while the steps of chunk c run on the
+ * main stream from buffer c%2, the forcing GEMM of chunk c+2 fills the other
+ * buffer on the forcing stream as soon as chunk c's steps are done, ordered by
+ * events. */
 typedef struct {
     Mat A;
-    int dir, prec;
+    int dir, prec, form;
     double h;
     PetscInt N, k, Nwe, twoNs, S;
     rsvdt_spmm_desc desc;
     void *y, *u1, *u2, *acc, *fbuf[2];
     void *ev_frc[2], *ev_rk[2];
     size_t esz;                  /* bytes per complex entry of the state */
-    const double *Fhat;          /* Nwe blocks, FP64 */
+    const double *Fin;           /* Nwe FP64 blocks: F_hat (form 1) or G (form 0) */
     const double *idft;          /* [2Ns][Nwe] FP64 complex */
 } stepper;
 
@@ -50,11 +71,11 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->val = dir ? (prec == 64 ? A->d_val64 : A->d_val32) : (prec == 64 ? A->d_valH64 : A->d_valH32);
     d->a_real = A->is_real;
     d->prec = prec;
-    const int p = prec == 32, q = dir ? 0 : 1;
-    if (A->tuned_ok[p][q] && nb == 1) {
-        d->lanes = A->tuned_lanes[p][q];
-        d->variant = A->tuned_variant[p][q];
-        d->xcd_remap = A->tuned_remap[p][q];
+    const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
+    if (A->tuned_ok[f][p][q] && nb == 1) {
+        d->lanes = A->tuned_lanes[f][p][q];
+        d->variant = A->tuned_variant[f][p][q];
+        d->xcd_remap = A->tuned_remap[f][p][q];
     } else {
         /* measured best on the 2D flow operator: 8-nonzero gather batches with
          * nontemporal row-local streams, lanes = k */
@@ -65,36 +86,41 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     if (g_rsvdt.opt.spmm_lanes) d->lanes = g_rsvdt.opt.spmm_lanes;
 }
 
-/* Steps per chunk: 2S+1 samples per chunk (both chunk ends included) keeps the
- * GEMM's row count at 63/31/15/7, just under the 64/32/16/8 MFMA row tiles. */
-static PetscInt auto_chunk(PetscInt N, PetscInt k, size_t esz)
+/* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
+ * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
+ * while the two chunk buffers would take more than a fifth of free memory. */
+static PetscInt chunk_samples(int form, PetscInt S) { return form ? 2 * S + 1 : S; }
+
+static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
 {
     if (g_rsvdt.opt.chunk_steps > 0) return g_rsvdt.opt.chunk_steps;
     size_t fr = 0, tot = 0;
     rsvdt_hip_mem_info(&fr, &tot);
     double blk = (double)N * (double)k * (double)esz;
-    PetscInt S = 31;
-    while (S > 3 && 2.0 * (2.0 * S + 1) * blk > 0.2 * (double)fr) S = (S - 1) / 2;
+    PetscInt S = form ? 31 : 64;
+    while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.2 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
     return S;
 }
 
 static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double h, PetscInt k, PetscInt Nwe,
-                                   PetscInt Ns, const double *Fhat, const double *idft)
+                                   PetscInt Ns, const double *Fin, const double *idft)
 {
     memset(st, 0, sizeof(*st));
     st->A = A; st->dir = dir; st->prec = prec; st->h = h;
+    st->form = g_rsvdt.opt.rk4_form ? 1 : 0;
     st->N = A->rows; st->k = k; st->Nwe = Nwe; st->twoNs = 2 * Ns;
     st->esz = prec == 64 ? 16 : 8;
-    st->Fhat = Fhat; st->idft = idft;
-    st->S = auto_chunk(st->N, k, st->esz);
+    st->Fin = Fin; st->idft = idft;
+    st->S = auto_chunk(st->form, st->N, k, st->esz);
     desc_for(&st->desc, A, dir, prec, k, 1);
     size_t blk = (size_t)st->N * (size_t)k * st->esz;
+    const size_t nsamp = (size_t)chunk_samples(st->form, st->S);
     st->y = rsvdt_scratch(SCR_Y, blk);
     st->u1 = rsvdt_scratch(SCR_U1, blk);
     st->u2 = rsvdt_scratch(SCR_U2, blk);
-    st->acc = rsvdt_scratch(SCR_ACC, blk);
-    st->fbuf[0] = rsvdt_scratch(SCR_FBUF, blk * (size_t)(2 * st->S + 1));
-    st->fbuf[1] = rsvdt_scratch(SCR_FBUF2, blk * (size_t)(2 * st->S + 1));
+    st->acc = st->form ? rsvdt_scratch(SCR_ACC, blk) : st->u2;
+    st->fbuf[0] = rsvdt_scratch(SCR_FBUF, blk * nsamp);
+    st->fbuf[1] = rsvdt_scratch(SCR_FBUF2, blk * nsamp);
     if (!st->y || !st->u1 || !st->u2 || !st->acc || !st->fbuf[0] || !st->fbuf[1])
         return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate the RK4 work buffers (%.2f GB each): %s",
                          blk / 1e9, rsvdt_hip_last_error());
@@ -125,32 +151,95 @@ static rsvdt_stream_t frc_stream(void)
     return g_rsvdt.opt.overlap_forcing ? g_rsvdt.stream_frc : g_rsvdt.stream;
 }
 
-/* Forcing samples of the chunk of steps i0 .. i0+steps-1 (1-based) into
- * buffer buf: slot j holds half-step position 2(i0-1)+j, j = 0..2 steps,
- * i.e. iDFT sample +pos (direct) or -pos (adjoint) mod 2Ns
- * (SourceCode/TSRK4.c:21-27, CreateForcingOnFly.c:17-19). */
+/* Forcing of the chunk of steps i0 .. i0+steps-1 (1-based) into buffer buf
+ * (CreateForcingOnFly.c:17-19, positions of TSRK4.c:21-27; the adjoint reads
+ * the iDFT table at -pos mod 2Ns):
+ *   form 1: slot j = half-step sample 2(i0-1)+j, j = 0 .. 2 steps, from F_hat;
+ *   form 0: slot j = g_{i0+j} = sum_m idft(m, 2(i0-1+j)) G_m. */
 static PetscErrorCode stepper_forcing(stepper *st, int buf, PetscInt i0, PetscInt steps)
 {
     const int64_t L = st->N * st->k;
     const int sgn = st->dir ? 1 : -1;
     const int64_t pos0 = 2 * (i0 - 1);
+    const int64_t P = st->form ? 2 * steps + 1 : steps;
+    const int stride = st->form ? sgn : 2 * sgn;
     RSVDT_HIPCHK("CreateForcingOnFly",
-                 rsvdt_hip_coef_gemm(2 * steps + 1, st->Nwe, L, st->idft, 1, st->Nwe, sgn * pos0, sgn, st->twoNs, 1.0,
-                                     st->Fhat, L, fslot(st, buf, 0), L, st->prec, 0.0, frc_stream()));
+                 rsvdt_hip_coef_gemm(P, st->Nwe, L, st->idft, 1, st->Nwe, sgn * pos0, stride, st->twoNs, 1.0, st->Fin,
+                                     L, fslot(st, buf, 0), L, st->prec, 0.0, frc_stream()));
     return 0;
 }
 
-/* One RK4 step (SourceCode/TSRK4.c:18-47) on the whole block, forcing from
- * slots 2s, 2s+1, 2s+2 of the chunk's buffer. */
+/* One RK4 step on the whole block; s = step index within the chunk. */
 static PetscErrorCode stepper_step(stepper *st, int buf, PetscInt s, double *snap)
 {
     const double h = st->h;
-    void *F0 = fslot(st, buf, 2 * s), *Fh = fslot(st, buf, 2 * s + 1), *F1 = fslot(st, buf, 2 * s + 2);
     rsvdt_stream_t q = g_rsvdt.stream;
+    if (st->form == 0) {
+        const rsvdt_spmm_desc *d = &st->desc;
+        RSVDT_HIPCHK("TSRK4", rsvdt_hip_horner_stage(d, st->y, st->y, NULL, NULL, h / 4, st->u1, NULL, q));
+        RSVDT_HIPCHK("TSRK4", rsvdt_hip_horner_stage(d, st->u1, st->y, NULL, NULL, h / 3, st->u2, NULL, q));
+        RSVDT_HIPCHK("TSRK4", rsvdt_hip_horner_stage(d, st->u2, st->y, NULL, NULL, h / 2, st->u1, NULL, q));
+        RSVDT_HIPCHK("TSRK4", rsvdt_hip_horner_stage(d, st->u1, st->y, NULL, fslot(st, buf, s), h, st->y, snap, q));
+        return 0;
+    }
+    void *F0 = fslot(st, buf, 2 * s), *Fh = fslot(st, buf, 2 * s + 1), *F1 = fslot(st, buf, 2 * s + 2);
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 1, st->y, st->y, F0, st->acc, st->u1, h / 2, h / 6, NULL, q));
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 2, st->u1, st->y, Fh, st->acc, st->u2, h / 2, h / 6 * 2, NULL, q));
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 3, st->u2, st->y, Fh, st->acc, st->u1, h, h / 6 * 2, NULL, q));
     RSVDT_HIPCHK("TSRK4", rsvdt_hip_rk4_stage(&st->desc, 4, st->u1, NULL, F1, st->acc, st->y, 0.0, h / 6, snap, q));
+    return 0;
+}
+
+/* G_m = Q(z_m) F_hat_m for the Nwe frequency blocks (form 0), FP64, written
+ * to G (which may be F itself: the last stage reads F row-locally only).
+ * z_m = Ns * idft(m, +-1) is the iDFT's own half-step phase of frequency m.
+ * Three Horner stages with per-block scales:
+ *   w1 = (1+z)/2 F + (h/4) A F;  w2 = (1+2z) F + h A w1;
+ *   G  = h/6 (1+4z+z^2) F + (h^2/6) A w2. */
+static PetscErrorCode folded_forcing(Mat A, int dir, double h, PetscInt k, PetscInt Nwe, PetscInt Ns,
+                                     const double *idft, const double *F, double *G)
+{
+    const PetscInt N = A->rows;
+    const size_t blk = (size_t)N * (size_t)k * 16;
+    double *zrow = malloc(sizeof(double) * 2 * (size_t)Nwe);
+    double *sc = malloc(sizeof(double) * 6 * (size_t)Nwe);
+    if (!zrow || !sc) { free(zrow); free(sc); return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "out of host memory"); }
+    const PetscInt j1 = dir ? 1 : 2 * Ns - 1;
+    int h1 = rsvdt_hip_memcpy_d2h(zrow, idft + 2 * (size_t)j1 * (size_t)Nwe, sizeof(double) * 2 * (size_t)Nwe);
+    if (h1) { free(zrow); free(sc); return rsvdt_err(h1, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error()); }
+    for (PetscInt m = 0; m < Nwe; m++) {
+        const double zr = (double)Ns * zrow[2 * m], zi = (double)Ns * zrow[2 * m + 1];
+        const double z2r = zr * zr - zi * zi, z2i = 2 * zr * zi;
+        sc[2 * m] = 0.5 * (1 + zr);                       sc[2 * m + 1] = 0.5 * zi;               /* (1+z)/2 */
+        sc[2 * (Nwe + m)] = 1 + 2 * zr;                   sc[2 * (Nwe + m) + 1] = 2 * zi;         /* 1+2z */
+        sc[2 * (2 * Nwe + m)] = h / 6 * (1 + 4 * zr + z2r); sc[2 * (2 * Nwe + m) + 1] = h / 6 * (4 * zi + z2i);
+    }
+    free(zrow);
+    double *dsc = rsvdt_scratch(SCR_FSCALE, sizeof(double) * 6 * (size_t)Nwe);
+    if (!dsc) { free(sc); return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate forcing scales"); }
+    int h2 = rsvdt_hip_memcpy_h2d(dsc, sc, sizeof(double) * 6 * (size_t)Nwe);
+    free(sc);
+    if (h2) return rsvdt_err(h2, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+    size_t fr = 0, tot = 0;
+    rsvdt_hip_mem_info(&fr, &tot);
+    PetscInt grp = Nwe;
+    while (grp > 1 && 2.0 * (double)grp * (double)blk > 0.25 * (double)fr) grp = (grp + 1) / 2;
+    double *w1 = rsvdt_scratch(SCR_W1, blk * (size_t)grp);
+    double *w2 = rsvdt_scratch(SCR_W2, blk * (size_t)grp);
+    if (!w1 || !w2) return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate forcing work buffers");
+    rsvdt_stream_t q = g_rsvdt.stream;
+    for (PetscInt b0 = 0; b0 < Nwe; b0 += grp) {
+        const PetscInt g = grp < Nwe - b0 ? grp : Nwe - b0;
+        rsvdt_spmm_desc d;
+        desc_for(&d, A, dir, 64, k, g);
+        d.lanes = 0;
+        const double *Fb = F + 2 * (size_t)b0 * (size_t)N * (size_t)k;
+        double *Gb = G + 2 * (size_t)b0 * (size_t)N * (size_t)k;
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, Fb, Fb, dsc + 2 * b0, NULL, h / 4, w1, NULL, q));
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, w1, Fb, dsc + 2 * (Nwe + b0), NULL, h, w2, NULL, q));
+        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, w2, Fb, dsc + 2 * (2 * Nwe + b0), NULL, h * h / 6, Gb,
+                                                           NULL, q));
+    }
     return 0;
 }
 
@@ -198,15 +287,16 @@ static void pt_free(phase_timer *t)
 }
 
 /* ------------------------------------------------------------------ SpMM autotune */
-/* Picks the launch shape of the fused SpMM+stage kernel for this operator,
- * precision and direction -- lanes per row (the column-block width a row's
- * lanes cover), gather batch / streaming variant and XCD row remap -- by
- * timing three stage-2 launches of every candidate on scratch buffers.  The
+/* Picks the launch shape of the fused SpMM stage kernel of the current
+ * stepping form for this operator, precision and direction -- lanes per row
+ * (the column-block width a row's lanes cover), gather batch / streaming
+ * variant and XCD row remap -- by timing three launches of every candidate
+ * on scratch buffers (a Horner stage, or stage 2 of the stage form).  The
  * choice is kept on the Mat and reused by every later action. */
 static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
 {
-    const int p = prec == 32, q = dir ? 0 : 1;
-    if (A->tuned_ok[p][q] || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
+    const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
+    if (A->tuned_ok[f][p][q] || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
     rsvdt_spmm_desc dsc;
     desc_for(&dsc, A, dir, prec, k, 1);
     void *e0, *e1;
@@ -221,10 +311,14 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         for (int v = 0; v < (cpl == 1 ? 4 : 1); v++)
             for (int r = 0; r < 2; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
-                /* stage 2 with uin = a, y = b, no forcing, acc = d, out = c: all scratch */
-                if (rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream)) continue;
+                /* all operands are scratch buffers: gathered a, row-local b (and d), out c */
+#define RSVDT_TUNE_LAUNCH()                                                                              \
+    (f ? rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream)                   \
+       : rsvdt_hip_horner_stage(&dsc, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream))
+                if (RSVDT_TUNE_LAUNCH()) continue;
                 rsvdt_hip_event_record(e0, g_rsvdt.stream);
-                for (int rep = 0; rep < 3; rep++) rsvdt_hip_rk4_stage(&dsc, 2, a, b, NULL, d, c, 0.0, 0.0, NULL, g_rsvdt.stream);
+                for (int rep = 0; rep < 3; rep++) (void)RSVDT_TUNE_LAUNCH();
+#undef RSVDT_TUNE_LAUNCH
                 rsvdt_hip_event_record(e1, g_rsvdt.stream);
                 float ms = 0;
                 rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
@@ -233,13 +327,13 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
     }
     rsvdt_hip_event_destroy(e0);
     rsvdt_hip_event_destroy(e1);
-    A->tuned_lanes[p][q] = bl;
-    A->tuned_variant[p][q] = bv;
-    A->tuned_remap[p][q] = br;
-    A->tuned_ok[p][q] = 1;
+    A->tuned_lanes[f][p][q] = bl;
+    A->tuned_variant[f][p][q] = bv;
+    A->tuned_remap[f][p][q] = br;
+    A->tuned_ok[f][p][q] = 1;
     if (g_rsvdt.opt.verbose)
-        rsvdt_printf("[rsvdt] autotune (%s, FP%d): %d lanes/row, variant %d, xcd remap %d: %.1f us per stage\n",
-                     dir ? "A" : "A^H", prec, bl, bv, br, 1e3 * best / 3);
+        rsvdt_printf("[rsvdt] autotune (%s, FP%d, %s): %d lanes/row, variant %d, xcd remap %d: %.1f us per stage\n",
+                     dir ? "A" : "A^H", prec, f ? "stage form" : "Horner form", bl, bv, br, 1e3 * best / 3);
     return 0;
 }
 
@@ -248,8 +342,8 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
  * rend = Nt + Ns (+RR) steps and stores Nstore snapshots, every RR steps
  * after Nt, into Y_all (block s = snapshot column s of every test vector;
  * the adjoint stores them reversed, SourceCode/SaveSnapshots.c:19-28). */
-static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat Fhat, Mat Yall, PetscInt max_steps,
-                                    phase_timer *t_rk4, phase_timer *t_frc)
+static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat Fhat, int fhat_folded, Mat Yall,
+                                    PetscInt max_steps, phase_timer *t_rk4, phase_timer *t_frc)
 {
     RSVDT_CHK(rsvdt_aij_upload(A));
     const int dir = V->TS.DirAdj ? 1 : 0;
@@ -261,9 +355,14 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     stepper st;
     RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
     if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes) {
-        RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.acc));
+        RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2));
         desc_for(&st.desc, A, dir, prec, k, 1);
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
+    }
+    if (st.form == 0 && !fhat_folded) {        /* G_m = Q(z_m) F_hat_m in place of F_hat */
+        if (t_frc) pt_begin(t_frc);
+        RSVDT_CHK(folded_forcing(A, dir, V->TS.dt, k, Nwe, V->TS.Ns, D->idft->d, Fhat->d, Fhat->d));
+        if (t_frc) pt_end(t_frc);
     }
     int64_t launches = 0;
     const PetscInt nch = (rend + st.S - 1) / st.S;
@@ -318,24 +417,28 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     g_rsvdt.stats.spmm_lanes = st.desc.lanes;
     g_rsvdt.stats.spmm_variant = st.desc.variant;
     g_rsvdt.stats.spmm_xcd_remap = st.desc.xcd_remap;
+    g_rsvdt.stats.rk4_form = st.form;
     return 0;
 }
 
 /* Homogeneous RK4 steps on nb blocks of N x k in place (SourceCode/TSTransRK4.c:15-36
- * via TSExpADeltaT.c:25-34), FP64, in groups that bound the temporaries. */
+ * via TSExpADeltaT.c:25-34), FP64, in groups that bound the temporaries;
+ * Horner stages (form 0) or the reference's stage order (form 1). */
 PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps, double *X, PetscInt N, PetscInt k,
                                        PetscInt nb)
 {
     RSVDT_CHK(rsvdt_aij_upload(A));
+    const int form = g_rsvdt.opt.rk4_form ? 1 : 0;
     size_t blk = (size_t)N * (size_t)k * 16;
     size_t fr = 0, tot = 0;
     rsvdt_hip_mem_info(&fr, &tot);
+    const double ntmp = form ? 3.0 : 2.0;
     PetscInt G = nb;
-    while (G > 1 && 3.0 * (double)G * (double)blk > 0.4 * (double)fr) G = (G + 1) / 2;
+    while (G > 1 && ntmp * (double)G * (double)blk > 0.4 * (double)fr) G = (G + 1) / 2;
     if (G > 65535) G = 65535;
     double *u1 = rsvdt_scratch(SCR_U1, blk * (size_t)G);
     double *u2 = rsvdt_scratch(SCR_U2, blk * (size_t)G);
-    double *acc = rsvdt_scratch(SCR_ACC, blk * (size_t)G);
+    double *acc = form ? rsvdt_scratch(SCR_ACC, blk * (size_t)G) : u2;
     if (!u1 || !u2 || !acc) return rsvdt_err(RSVDT_ERR_MEM, "TSExpADeltaT", "cannot allocate work buffers");
     for (PetscInt g0 = 0; g0 < nb; g0 += G) {
         PetscInt g = G < nb - g0 ? G : nb - g0;
@@ -345,6 +448,13 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         double *y = X + 2 * (size_t)g0 * (size_t)N * (size_t)k;
         rsvdt_stream_t q = g_rsvdt.stream;
         for (PetscInt it = 0; it < nsteps; it++) {
+            if (form == 0) {
+                RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_stage(&d, y, y, NULL, NULL, h / 4, u1, NULL, q));
+                RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_stage(&d, u1, y, NULL, NULL, h / 3, u2, NULL, q));
+                RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_stage(&d, u2, y, NULL, NULL, h / 2, u1, NULL, q));
+                RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_stage(&d, u1, y, NULL, NULL, h, y, NULL, q));
+                continue;
+            }
             RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 1, y, y, NULL, acc, u1, h / 2, h / 6, NULL, q));
             RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 2, u1, y, NULL, acc, u2, h / 2, h / 6 * 2, NULL, q));
             RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_rk4_stage(&d, 3, u2, y, NULL, acc, u1, h, h / 6 * 2, NULL, q));
@@ -702,7 +812,7 @@ PetscErrorCode TSActionRK4(RSVD_matrices *R, DFT_matrices *D, LNS_vars *LNS, RSV
     Mat Yall;
     RSVDT_CHK(rsvdt_mat_alloc_dense(N, Nstore * k, k, 1, &Yall));
     phase_timer trk = {0}, tfr = {0}, tdf = {0};
-    PetscErrorCode ierr = time_stepping(LNS->A, D, V, R->F_hat, Yall, 0, &trk, &tfr);
+    PetscErrorCode ierr = time_stepping(LNS->A, D, V, R->F_hat, 0, Yall, 0, &trk, &tfr);
     RSVDT_MatDestroy(&R->F_hat);                              /* :139 */
     if (ierr) { RSVDT_MatDestroy(&Yall); return ierr; }
     pt_begin(&tdf);
@@ -835,17 +945,36 @@ PetscErrorCode RSVDT_BenchSteps(RSVD_matrices *R, RSVDt_vars *V, LNS_vars *LNS, 
     Mat F = R->Y_hat;
     PetscInt k = V->RSVD.k;
     if (F->order != 0 || F->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(F, k, 0));
-    if (warmup > 0) RSVDT_CHK(time_stepping(LNS->A, D, V, F, NULL, warmup, NULL, NULL));
-    RSVDT_HIPCHK("RSVDT_BenchSteps", rsvdt_hip_sync());
+    /* the folded forcing terms are computed once per action; here once,
+     * outside the timed steps, into a copy so that Y_hat stays the sketch */
+    Mat Fin = F;
+    int folded = 0;
+    if (!g_rsvdt.opt.rk4_form) {
+        RSVDT_CHK(rsvdt_aij_upload(LNS->A));
+        RSVDT_CHK(rsvdt_mat_alloc_dense(F->rows, F->cols, F->kb, F->order, &Fin));
+        PetscErrorCode e = folded_forcing(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, k, V->RSVD.Nw_eff, V->TS.Ns,
+                                          D->idft->d, F->d, Fin->d);
+        if (e) { RSVDT_MatDestroy(&Fin); return e; }
+        folded = 1;
+    }
+    PetscErrorCode ierr = 0;
+    if (warmup > 0) ierr = time_stepping(LNS->A, D, V, Fin, folded, NULL, warmup, NULL, NULL);
+    if (!ierr) {
+        int hs = rsvdt_hip_sync();
+        if (hs) ierr = rsvdt_err(hs, "RSVDT_BenchSteps", "device call failed: %s", rsvdt_hip_last_error());
+    }
     phase_timer trk = {0}, tfr = {0}, tall = {0};
-    pt_begin(&tall);
-    PetscErrorCode ierr = time_stepping(LNS->A, D, V, F, NULL, nsteps, &trk, &tfr);
-    pt_end(&tall);
-    RSVDT_HIPCHK("RSVDT_BenchSteps", rsvdt_hip_sync());
-    pt_collect(&trk); pt_collect(&tfr); pt_collect(&tall);
-    if (ms_out) *ms_out = tall.ms;
-    if (rk4_ms_out) *rk4_ms_out = trk.ms;
-    if (forcing_ms_out) *forcing_ms_out = tfr.ms;
+    if (!ierr) {
+        pt_begin(&tall);
+        ierr = time_stepping(LNS->A, D, V, Fin, folded, NULL, nsteps, &trk, &tfr);
+        pt_end(&tall);
+        rsvdt_hip_sync();
+        pt_collect(&trk); pt_collect(&tfr); pt_collect(&tall);
+        if (ms_out) *ms_out = tall.ms;
+        if (rk4_ms_out) *rk4_ms_out = trk.ms;
+        if (forcing_ms_out) *forcing_ms_out = tfr.ms;
+    }
     pt_free(&trk); pt_free(&tfr); pt_free(&tall);
+    if (Fin != F) RSVDT_MatDestroy(&Fin);
     return ierr;
 }

@@ -35,9 +35,10 @@ struct _p_RSVDT_Mat {
     double *h_val;           /* 2*nnz */
     int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
     void *d_val64, *d_val32, *d_valH64, *d_valH32;
-    /* autotuned SpMM launch shape per [precision 64/32][direct/adjoint]:
-     * lanes per row, kernel variant, XCD remap; tuned_ok marks a tuned entry */
-    int tuned_lanes[2][2], tuned_variant[2][2], tuned_remap[2][2], tuned_ok[2][2];
+    /* autotuned SpMM launch shape per [stepping form][precision 64/32]
+     * [direct/adjoint]: lanes per row, kernel variant, XCD remap; tuned_ok
+     * marks a tuned entry */
+    int tuned_lanes[2][2][2], tuned_variant[2][2][2], tuned_remap[2][2][2], tuned_ok[2][2][2];
 };
 
 struct _p_RSVDT_Vec {

@@ -199,6 +199,7 @@ static void default_options(RSVDT_Options *o)
     o->autotune = 1;
     o->verbose = 0;
     o->overlap_forcing = 1;
+    o->rk4_form = 0;
 }
 
 PetscErrorCode RSVDT_Initialize(int *argc, char ***argv)

@@ -59,7 +59,7 @@ static PetscErrorCode read_flag_dir(const char *flag, const char *dirkey, PetscB
 
 /* SourceCode/ReadUserInput.c:5-175.  Extension keys (no counterpart in the
  * reference): Precision (fp64|fp32), ForcingChunk, SpMMLanes, UseGraph,
- * Autotune, XCDRemap, Verbose. */
+ * Autotune, XCDRemap, Verbose, RK4Form (horner|stages). */
 PetscErrorCode ReadUserInput(RSVDt_vars *RSVDt, Weight_matrices *W, LNS_vars *LNS, Resolvent_matrices *Res,
                              TransRun_vars *TR, Directories *dirs)
 {
@@ -144,6 +144,11 @@ PetscErrorCode ReadUserInput(RSVDt_vars *RSVDt, Weight_matrices *W, LNS_vars *LN
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Autotune", &b, &ok) && ok) o.autotune = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("XCDRemap", &b, &ok) && ok) o.xcd_remap = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Verbose", &b, &ok) && ok) o.verbose = b; }
+    if (rsvdt_opt_get_string("RK4Form", s, sizeof(s))) {
+        if (!strcmp(s, "horner") || !strcmp(s, "0")) o.rk4_form = 0;
+        else if (!strcmp(s, "stages") || !strcmp(s, "1")) o.rk4_form = 1;
+        else return rsvdt_err(RSVDT_ERR_USER, "ReadUserInput", "'RK4Form' must be horner or stages");
+    }
     RSVDT_CHK(RSVDT_SetOptions(&o));
     rsvdt_printf("\n");
     return 0;

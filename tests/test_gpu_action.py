@@ -24,8 +24,10 @@ CASES = [
 
 @pytest.mark.parametrize("N,Nw,cplx,tr,TL,k", CASES)
 @pytest.mark.parametrize("direct", [True, False])
-def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct):
-    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct}", flush=True)
+@pytest.mark.parametrize("form", [0, 1])
+def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form):
+    """form 0: Horner stages with the folded forcing; form 1: the reference's stage order."""
+    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct} form={form}", flush=True)
     A = kat.gl_operator(N=N, L=20.0)
     if not cplx:
         A = A.real.astype(complex)
@@ -33,7 +35,7 @@ def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct):
     o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
     F = crandn(rng, N, k * o.Nw_eff)
     Yo = o.action(F, direct=direct)
-    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr, rk4_form=form)
     assert (s.Ns, s.Nt, s.RR) == (o.Ns, o.Nt, o.RR)
     s.set_sketch(F)
     s.direct() if direct else s.adjoint()
@@ -49,10 +51,11 @@ def test_chunk_boundaries_and_lanes(pkg):
     o = oracle.Oracle(A, k=4, q=0, Nw=8, w=0.3, TL=1.3, dt=0.02, transient_removal=False)
     F = crandn(rng, 48, 4 * o.Nw_eff)
     Yo = o.action(F)
-    for chunk, lanes in [(1, 4), (3, 8), (7, 16), (32, 32), (5, 64)]:
-        print(f"chunk={chunk} lanes={lanes}", flush=True)
+    for chunk, lanes, form in [(1, 4, 0), (3, 8, 0), (7, 16, 0), (32, 32, 0), (5, 64, 0), (1, 4, 1), (3, 8, 1),
+                               (7, 16, 1), (32, 32, 1), (5, 64, 1)]:
+        print(f"chunk={chunk} lanes={lanes} form={form}", flush=True)
         s = pkg.Session(A, k=4, q=0, Nw=8, w=0.3, TL=1.3, dt=0.02, transient_removal=False, chunk_steps=chunk,
-                        spmm_lanes=lanes)
+                        spmm_lanes=lanes, rk4_form=form)
         s.set_sketch(F)
         s.direct()
         assert rel(s.get_Y(), Yo) < 1e-10

@@ -176,3 +176,48 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
     tol = 1e-13 if prec == 64 else 3e-6
     assert rel(out, ref) < tol
     assert np.array_equal(snap, out.astype(np.complex128))
+
+
+@pytest.mark.parametrize("k,real,prec,nb,lanes,variant,scaled,forced", [
+    (1, False, 64, 1, 0, 0, False, True), (16, True, 64, 1, 0, 3, False, True), (16, True, 64, 1, 16, 1, True, False),
+    (10, False, 64, 3, 0, 0, True, True), (16, False, 64, 2, 8, 0, False, False), (32, True, 64, 1, 0, 2, False, True),
+    (40, False, 64, 1, 0, 0, True, False), (16, True, 32, 1, 0, 3, False, True), (16, True, 32, 1, 8, 0, False, True),
+    (10, False, 32, 2, 4, 0, True, True)])
+def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced):
+    """out_b = ys_b y_b + c A vin_b + g_b, with the snapshot copy, against numpy."""
+    print(f"horner k={k} real={real} prec={prec} nb={nb} lanes={lanes} variant={variant} scaled={scaled} "
+          f"forced={forced}", flush=True)
+    rng = np.random.default_rng(7 * k + nb)
+    N = 2999
+    A = sp.random(N, N, density=6.0 / N, random_state=k + 3, format="csr", dtype=np.float64)
+    if not real:
+        A = A + 1j * sp.random(N, N, density=3.0 / N, random_state=k + 4, format="csr")
+    A = (A - sp.eye(N)).tocsr().astype(np.complex128)
+    A.sort_indices()
+    c = 0.037
+    vin, y, g = crandn(rng, nb, N, k), crandn(rng, nb, N, k), crandn(rng, nb, N, k)
+    ys = crandn(rng, nb) if scaled else np.ones(nb, complex)
+    ref = np.stack([ys[b] * y[b] + c * (A @ vin[b]) + (g[b] if forced else 0) for b in range(nb)])
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rdt = np.float64 if prec == 64 else np.float32
+    vals = A.data.real.astype(rdt) if real else A.data.astype(cdt)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
+    dvin, dy, dg = Dev(pkg, vin.astype(cdt)), Dev(pkg, y.astype(cdt)), Dev(pkg, g.astype(cdt))
+    dys = Dev(pkg, ys.astype(np.complex128))
+    dsnap = Dev(pkg, nbytes=y.nbytes)
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, lanes, 1, variant)
+    L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
+    # out aliases y (the in-place last stage of a step)
+    rc = L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, dys.p if scaled else None, dg.p if forced else None,
+                                  c, dy.p, dsnap.p, None)
+    assert rc == 0
+    out = dy.get(cdt, y.shape)
+    snap = dsnap.get(np.complex128, y.shape)
+    assert rel(out, ref) < (1e-14 if prec == 64 else 2e-6)
+    assert np.array_equal(snap, out.astype(np.complex128))
+
+
+def test_horner_stage_rejects_bad_desc(L, pkg):
+    d = Desc(0, 16, 1, None, None, None, 1, 64, 0, 1, 0)
+    L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
+    assert L.rsvdt_hip_horner_stage(ctypes.byref(d), None, None, None, None, 1.0, None, None, None) != 0
