@@ -199,6 +199,13 @@ int rsvdt_hip_zcopy3(int64_t nb, int64_t nr, int64_t nc,
  * c*rows + r (uniform [0,1) + i[0,1) when gauss = 0, complex N(0,1) when 1). */
 int rsvdt_hip_fill_random(int64_t rows, int64_t kb, int64_t nblk, uint64_t seed, int gauss, double *out,
                           rsvdt_stream_t s);
+/* Per-row transpose between the block layout [s][r][b] (n blocks, block s
+ * an nrows x kt row-major slab) and the per-batch row-major layout [j][r][s]
+ * (kb batches, batch j an nrows x n row-major matrix), for the batches
+ * b0 .. b0+kb-1 of the block layout.  dir 0: block -> per-batch;
+ * dir 1: per-batch -> block.  n*(kb+1) <= 8192.                           */
+int rsvdt_hip_transpose_rows(int64_t nrows, int64_t n, int64_t kt, int64_t b0, int64_t kb, const double *src,
+                             double *dst, int dir, rsvdt_stream_t s);
 /* sum of |x|^2 over n complex entries into *out (device double) */
 int rsvdt_hip_znorm2(int64_t n, const double *x, double *out_dev, void *work, rsvdt_stream_t s);
 

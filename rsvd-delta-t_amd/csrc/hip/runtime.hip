@@ -207,6 +207,54 @@ extern "C" int rsvdt_hip_zcopy3(int64_t nb, int64_t nr, int64_t nc, const double
     return RSVDT_HIP_OK;
 }
 
+// Per-row transpose between the block layout [s][r][b] (block s = an
+// nrows x kt row-major slab) and the per-batch row-major layout [j][r][s]
+// (batch j = nrows x n row-major), for the batch range b0 .. b0+kb-1.  One
+// workgroup per row; both sides are read and written in contiguous runs.
+__global__ __launch_bounds__(256) void k_transpose_rows(int64_t nrows, int n, int64_t kt, int64_t b0, int kb,
+                                                        const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                        int dir)
+{
+    extern __shared__ double2 tt[];   // [n][kb+1]
+    const int64_t r = blockIdx.x;
+    const int tot = n * kb;
+    if (dir == 0) {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int s = e / kb, j = e % kb;
+            tt[s * (kb + 1) + j] = src[(int64_t)s * nrows * kt + r * kt + b0 + j];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int j = e / n, s = e % n;
+            dst[(int64_t)j * nrows * n + r * n + s] = tt[s * (kb + 1) + j];
+        }
+    } else {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int j = e / n, s = e % n;
+            tt[s * (kb + 1) + j] = src[(int64_t)j * nrows * n + r * n + s];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int s = e / kb, j = e % kb;
+            dst[(int64_t)s * nrows * kt + r * kt + b0 + j] = tt[s * (kb + 1) + j];
+        }
+    }
+}
+extern "C" int rsvdt_hip_transpose_rows(int64_t nrows, int64_t n, int64_t kt, int64_t b0, int64_t kb,
+                                        const double *src, double *dst, int dir, rsvdt_stream_t s)
+{
+    if (nrows <= 0 || n <= 0 || kb <= 0) return RSVDT_HIP_OK;
+    if (b0 < 0 || b0 + kb > kt || nrows > 0x7fffffffLL || n * (kb + 1) > 8192 || (dir != 0 && dir != 1)) {
+        set_error_msg("rsvdt_hip_transpose_rows: bad arguments (n*(kb+1) <= 8192, b0+kb <= kt)");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    const size_t lds = (size_t)n * (size_t)(kb + 1) * 16;
+    hipLaunchKernelGGL(k_transpose_rows, dim3((unsigned)nrows), dim3(256), lds, as_stream(s), nrows, (int)n, kt, b0,
+                       (int)kb, (const double2 *)src, (double2 *)dst, dir);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_transpose_rows");
+    return RSVDT_HIP_OK;
+}
+
 // sum |x|^2: per-block partials into work, then one block sums them (deterministic)
 __global__ void k_znorm2_part(int64_t n, const double2 *x, double *part) {
     __shared__ double sh[256];

@@ -18,7 +18,7 @@
 
 /* scratch slots */
 enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_FBUF2, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP,
-       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_FSCALE, SCR_W1, SCR_W2, SCR_NSLOTS };
+       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_FSCALE, SCR_W1, SCR_W2, SCR_QRT, SCR_GINFO, SCR_NSLOTS };
 _Static_assert(SCR_NSLOTS <= RSVDT_NSCRATCH, "scratch slot table too small");
 
 /* ------------------------------------------------------------------ stepper */
@@ -469,11 +469,16 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
  * pass on X^H X + s I with s = 11(mn + n(n+1)) u ||X||_F^2, then CholeskyQR2.
  * Q is X R^{-1} with R = R3 R2 R1 upper triangular with a positive real
  * diagonal -- the unique factor that SLEPc's Gram-Schmidt BVOrthogonalize
- * also returns for a full-rank input (SourceCode/QRAllFreqs.c:26). */
-/* npass CholeskyQR passes in place (the first one shifted when shift_first);
- * R_out (optional) receives the product of the passes' factors. */
+ * also returns for a full-rank input (SourceCode/QRAllFreqs.c:26).
+ *
+ * npass passes on X (the first one shifted when shift_first); R_out
+ * (optional) receives the product of the passes' factors.  T (optional): a
+ * buffer like X of `bytes` bytes; the applies then alternate between X and
+ * T (required for more than 64 columns with unit column strides, where the
+ * apply cannot work in place) and the result is copied back into X. */
 static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
-                                    int64_t xa, double *R_out, int npass, int shift_first, double drop_tol)
+                                    int64_t xa, double *R_out, int npass, int shift_first, double drop_tol,
+                                    double *T, size_t bytes)
 {
     size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
     double *G = rsvdt_scratch(SCR_G, mat);
@@ -488,11 +493,14 @@ static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, dou
     rsvdt_stream_t q = g_rsvdt.stream;
     RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_zero(info, 256, q));
     const double shift_rel = 11.0 * ((double)nrows * (double)n + (double)n * (double)(n + 1)) * 1.1102230246251565e-16;
+    double *cur = X, *oth = T ? T : X;
     for (int pass = 0; pass < npass; pass++) {
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, X, xb, xr, xa, X, xb, xr, xa, G, work, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, cur, xb, xr, xa, cur, xb, xr, xa, G, work, q));
         RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, G, (pass == 0 && shift_first) ? shift_rel : 0.0, drop_tol,
                                                        R, Ri, info, q));
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, X, xb, xr, xa, Ri, 0.0, NULL, 0, 0, 0, X, xb, xr, xa, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, cur, xb, xr, xa, Ri, 0.0, NULL, 0, 0, 0, oth, xb, xr,
+                                                    xa, q));
+        if (T) { double *tmp = cur; cur = oth; oth = tmp; }
         if (R_out) {
             if (pass == 0) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, R, mat, q));
             else {
@@ -501,6 +509,7 @@ static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, dou
             }
         }
     }
+    if (cur != X) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(X, cur, bytes, q));
     if (R_out) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(R_out, Ra, mat, q));
     int hinfo = 0;
     RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)));
@@ -511,11 +520,20 @@ static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, dou
 PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,
                             double *R_out, double drop_tol)
 {
-    return cholqr_passes(nb, nrows, n, X, xb, xr, xa, R_out, 3, 1, drop_tol);
+    double *T = NULL;
+    size_t bytes = (size_t)nb * (size_t)nrows * (size_t)n * 16;
+    if (n > 64 && xa == 1) {      /* the wide apply is out of place: a dense contiguous layout is required */
+        const int64_t ext = (nb - 1) * xb + (nrows - 1) * xr + (n - 1) * xa + 1;
+        if (ext != nb * nrows * n)
+            return rsvdt_err(RSVDT_ERR_USER, "QRAllFreqs", "more than 64 columns need a contiguous layout");
+        T = rsvdt_scratch(SCR_QRT, bytes);
+        if (!T) return rsvdt_err(RSVDT_ERR_MEM, "QRAllFreqs", "cannot allocate CholeskyQR work");
+    }
+    return cholqr_passes(nb, nrows, n, X, xb, xr, xa, R_out, 3, 1, drop_tol, T, bytes);
 }
 
 PetscErrorCode rsvdt_span_basis(PetscInt nb, PetscInt nrows, PetscInt n, const double *X, double *Q, double *E,
-                                int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol)
+                                double *T, int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol)
 {
     size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
     double *C = rsvdt_scratch(SCR_PROJ, mat);
@@ -528,7 +546,7 @@ PetscErrorCode rsvdt_span_basis(PetscInt nb, PetscInt nrows, PetscInt n, const d
     const int64_t nel = (int64_t)(bytes / 16);
     /* level 1 */
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_memcpy_d2d(Q, X, bytes, q));
-    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, Rm, 3, 1, drop_tol));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, Rm, 3, 1, drop_tol, T, bytes));
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zero(cnt, 256, q));
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bdrop_mask(nb, n, Rm, D, cnt, q));
     int ndrop = 0;
@@ -537,17 +555,17 @@ PetscErrorCode rsvdt_span_basis(PetscInt nb, PetscInt nrows, PetscInt n, const d
     /* level 2: E = ((I - Q Q^H) X) D, the residuals of the dropped columns */
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(nb, nrows, n, n, Q, xb, xr, xa, X, xb, xr, xa, C, work, q));
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zscale((int64_t)nb * n * n, -1.0, C, C, q));
-    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, X, xb, xr, xa, E, xb, xr, xa, q));
-    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, E, xb, xr, xa, D, 0.0, NULL, 0, 0, 0, E, xb, xr, xa, q));
-    RSVDT_CHK(cholqr_passes(nb, nrows, n, E, xb, xr, xa, NULL, 3, 1, drop_tol));
-    /* reorthogonalise against the level-1 basis and renormalise */
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, X, xb, xr, xa, T, xb, xr, xa, q));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, T, xb, xr, xa, D, 0.0, NULL, 0, 0, 0, E, xb, xr, xa, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, E, xb, xr, xa, NULL, 3, 1, drop_tol, T, bytes));
+    /* reorthogonalise against the level-1 basis (into T) and renormalise */
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(nb, nrows, n, n, Q, xb, xr, xa, E, xb, xr, xa, C, work, q));
     RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zscale((int64_t)nb * n * n, -1.0, C, C, q));
-    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, E, xb, xr, xa, E, xb, xr, xa, q));
-    RSVDT_CHK(cholqr_passes(nb, nrows, n, E, xb, xr, xa, NULL, 2, 0, drop_tol));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bapply(nb, nrows, n, n, Q, xb, xr, xa, C, 1.0, E, xb, xr, xa, T, xb, xr, xa, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, T, xb, xr, xa, NULL, 2, 0, drop_tol, E, bytes));
     /* the level-2 columns take the dropped columns' places; one more pass */
-    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zaxpy(nel, 1.0, E, Q, q));
-    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, NULL, 1, 0, drop_tol));
+    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_zaxpy(nel, 1.0, T, Q, q));
+    RSVDT_CHK(cholqr_passes(nb, nrows, n, Q, xb, xr, xa, NULL, 1, 0, drop_tol, T, bytes));
     return 0;
 }
 
@@ -610,33 +628,75 @@ PetscErrorCode DFT(Mat Yall, DFT_matrices *D, RSVD_matrices *R, RSVDt_vars *V)
 }
 
 /* ------------------------------------------------------------------ transient removal */
-/* QR of the snapshot blocks, shared by CreateMtilde and GalerkinProjAllModes
- * (the reference computes the identical factor twice,
- * SourceCode/CreateMtilde.c:43-46 and GalerkinProjAllModes.c:31-33). */
-static Mat g_trV = NULL;
-static const void *g_trV_key = NULL;
+/* The removal is independent per test vector, so it works on the
+ * per-test-vector row-major layout: X_ik (N x Ns1, row r = the Ns1
+ * snapshots of row r) for a group of test vectors at a time, sized to the
+ * free memory.  In that layout the snapshot basis, M_tilde and the Galerkin
+ * sums are batched tall-skinny products with contiguous rows, and the RR
+ * homogeneous steps of the basis are block SpMMs with Ns1 columns.
+ * When one group holds all test vectors, the snapshots and the basis built
+ * by CreateMtilde are kept for GalerkinProjAllModes (the reference computes
+ * the same QR in both, SourceCode/CreateMtilde.c:43-46 and
+ * GalerkinProjAllModes.c:31-33); otherwise each recomputes it per group. */
+typedef struct {
+    const void *key;             /* Y_all data the cached group was built from */
+    PetscInt b0, kb, N, n;
+    double *X, *V;               /* kb x N x n each */
+} tr_group;
+static tr_group g_trg = {0};
 
-static PetscErrorCode snapshot_basis(Mat Yall, PetscInt k, Mat *Vout)
+static void tr_group_free(tr_group *g)
 {
-    if (g_trV && g_trV_key == (const void *)Yall->d) { *Vout = g_trV; return 0; }
-    RSVDT_MatDestroy(&g_trV);
-    PetscInt Ns1 = Yall->cols / k, N = Yall->rows;
+    if (g->X || g->V) rsvdt_hip_sync();
+    if (g->X) rsvdt_hip_free(g->X);
+    if (g->V) rsvdt_hip_free(g->V);
+    memset(g, 0, sizeof(*g));
+}
+
+/* test vectors per group: X, V, two basis work arrays, M V and two stepping
+ * temporaries per test vector must fit in half of the free memory */
+static PetscInt tr_group_size(PetscInt k, PetscInt N, PetscInt n)
+{
+    size_t fr = 0, tot = 0;
+    rsvdt_hip_mem_info(&fr, &tot);
+    const double per = 7.0 * (double)N * (double)n * 16.0;
+    PetscInt kb = (PetscInt)(0.5 * (double)fr / per);
+    if (kb < 1) kb = 1;
+    if (kb > k) kb = k;
+    const PetscInt ng = (k + kb - 1) / kb;
+    return (k + ng - 1) / ng;               /* balanced groups */
+}
+
+/* Snapshots (relaid) and orthonormal basis of test vectors b0 .. b0+kb-1. */
+static PetscErrorCode tr_group_build(Mat Yall, PetscInt k, PetscInt b0, PetscInt kb, tr_group *g)
+{
+    const PetscInt N = Yall->rows, n = Yall->cols / k;
+    if (g->key == (const void *)Yall->d && g->b0 == b0 && g->kb == kb && g->X && g->V) return 0;
+    tr_group_free(g);
     if (Yall->order != 1 || Yall->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(Yall, k, 1));
-    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Yall->cols, k, 1, &g_trV));
-    Mat E;
-    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Yall->cols, k, 1, &E));
-    /* batch = test vector ik: X_ik(r, s) = Y_all[s*N*k + r*k + ik].  The
-     * snapshots of one period are a periodic response of rank <= Nw_eff plus
-     * a transient that decays, so the snapshot matrix is often numerically
-     * rank-deficient (singular values down to rounding level); the basis
-     * spans it to rounding accuracy with the numerically dependent
-     * directions left out (zero columns). */
-    PetscErrorCode ierr = rsvdt_span_basis(k, N, Ns1, Yall->d, g_trV->d, E->d, 1, k, N * k,
-                                           (size_t)N * (size_t)Yall->cols * 16, 1e-14);
-    RSVDT_MatDestroy(&E);
-    RSVDT_CHK(ierr);
-    g_trV_key = Yall->d;
-    *Vout = g_trV;
+    const size_t bytes = (size_t)kb * (size_t)N * (size_t)n * 16;
+    double *E = NULL, *T = NULL;
+    if (rsvdt_hip_malloc((void **)&g->X, bytes) || rsvdt_hip_malloc((void **)&g->V, bytes) ||
+        rsvdt_hip_malloc((void **)&E, bytes) || rsvdt_hip_malloc((void **)&T, bytes)) {
+        if (E) rsvdt_hip_free(E);
+        tr_group_free(g);
+        return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate the snapshot basis (%.2f GB each)", bytes / 1e9);
+    }
+    g->b0 = b0; g->kb = kb; g->N = N; g->n = n;
+    PetscErrorCode ierr = 0;
+    int h = rsvdt_hip_transpose_rows(N, n, k, b0, kb, Yall->d, g->X, 0, g_rsvdt.stream);
+    if (h) ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
+    /* The snapshots of one period are a periodic response of rank <= Nw_eff
+     * plus a decaying transient, so X_ik is often numerically rank-deficient
+     * (singular values down to rounding level); the basis spans it to
+     * rounding accuracy with the numerically dependent directions left out
+     * (zero columns). */
+    if (!ierr) ierr = rsvdt_span_basis(kb, N, n, g->X, g->V, E, T, N * n, n, 1, bytes, 1e-14);
+    rsvdt_hip_sync();
+    rsvdt_hip_free(E);
+    rsvdt_hip_free(T);
+    if (ierr) { tr_group_free(g); return ierr; }
+    g->key = Yall->d;
     return 0;
 }
 
@@ -645,18 +705,42 @@ static PetscErrorCode snapshot_basis(Mat Yall, PetscInt k, Mat *Vout)
 PetscErrorCode CreateMtilde(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR)
 {
     PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1, N = Yall->rows;
-    Mat Vb;
-    RSVDT_CHK(snapshot_basis(Yall, k, &Vb));
-    Mat MV;
-    RSVDT_CHK(rsvdt_mat_dup(Vb, &MV));
-    RSVDT_CHK(rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, MV->d, N, k, Ns1));
+    if (Yall->cols != Ns1 * k) return rsvdt_err(RSVDT_ERR_USER, "CreateMtilde", "Y_all must have (Nw+1)*k columns");
+    RSVDT_CHK(rsvdt_aij_upload(LNS->A));
+    RSVDT_MatDestroy(&TSR->M_tilde_all);
     RSVDT_CHK(rsvdt_mat_alloc_dense(Ns1, Ns1 * k, Ns1, 0, &TSR->M_tilde_all));
-    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(k, N, Ns1, Ns1));
-    if (!work) return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate work");
-    RSVDT_HIPCHK("CreateMtilde", rsvdt_hip_bgram(k, N, Ns1, Ns1, Vb->d, 1, k, N * k, MV->d, 1, k, N * k,
-                                                 TSR->M_tilde_all->d, work, g_rsvdt.stream));
-    RSVDT_MatDestroy(&MV);
-    return 0;
+    const PetscInt kb = tr_group_size(k, N, Ns1);
+    const size_t gbytes = (size_t)kb * (size_t)N * (size_t)Ns1 * 16;
+    double *MV = NULL, *Vblk = NULL;
+    if (rsvdt_hip_malloc((void **)&MV, gbytes) || rsvdt_hip_malloc((void **)&Vblk, gbytes)) {
+        if (MV) rsvdt_hip_free(MV);
+        return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate M V (%.2f GB)", gbytes / 1e9);
+    }
+    PetscErrorCode ierr = 0;
+    for (PetscInt b0 = 0; b0 < k && !ierr; b0 += kb) {
+        const PetscInt g = kb < k - b0 ? kb : k - b0;
+        ierr = tr_group_build(Yall, k, b0, g, &g_trg);
+        if (ierr) break;
+        /* the group's basis vectors as Ns1 blocks of N x g (the snapshot
+         * layout), stepped by the same k-column SpMM as the actions */
+        int h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, g_trg.V, Vblk, 1, g_rsvdt.stream);
+        if (h) { ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error()); break; }
+        ierr = rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, Vblk, N, g, Ns1);
+        if (ierr) break;
+        h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, Vblk, MV, 0, g_rsvdt.stream);
+        if (h) { ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error()); break; }
+        void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
+        if (!work) { ierr = rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate work"); break; }
+        h = rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, N * Ns1, Ns1, 1, MV, N * Ns1, Ns1, 1,
+                            TSR->M_tilde_all->d + 2 * (size_t)b0 * (size_t)Ns1 * (size_t)Ns1, work, g_rsvdt.stream);
+        if (h) ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
+        if (kb < k) tr_group_free(&g_trg);
+    }
+    rsvdt_hip_sync();
+    rsvdt_hip_free(MV);
+    rsvdt_hip_free(Vblk);
+    if (ierr) tr_group_free(&g_trg);
+    return ierr;
 }
 
 /* SourceCode/TSExpADeltaT.c:25-37: U <- e^{A dT} U column by column (RR
@@ -679,14 +763,22 @@ PetscErrorCode QRDecomposition(Mat Vm)
 {
     if (Vm->kind != RSVDT_MAT_DENSE) return rsvdt_err(RSVDT_ERR_USER, "QRDecomposition", "dense Mat required");
     if (Vm->kb != Vm->cols) RSVDT_CHK(rsvdt_mat_to_layout(Vm, Vm->cols, 0));
-    Mat X = NULL, E = NULL;
-    RSVDT_CHK(rsvdt_mat_dup(Vm, &X));
-    PetscErrorCode ierr = rsvdt_mat_alloc_dense(Vm->rows, Vm->cols, Vm->cols, 0, &E);
+    const size_t bytes = (size_t)Vm->rows * (size_t)Vm->cols * 16;
+    double *X = NULL, *E = NULL, *T = NULL;
+    PetscErrorCode ierr = 0;
+    if (rsvdt_hip_malloc((void **)&X, bytes) || rsvdt_hip_malloc((void **)&E, bytes) ||
+        rsvdt_hip_malloc((void **)&T, bytes))
+        ierr = rsvdt_err(RSVDT_ERR_MEM, "QRDecomposition", "cannot allocate work");
+    if (!ierr) {
+        int h = rsvdt_hip_memcpy_d2d(X, Vm->d, bytes, g_rsvdt.stream);
+        if (h) ierr = rsvdt_err(h, "QRDecomposition", "device call failed: %s", rsvdt_hip_last_error());
+    }
     if (!ierr)
-        ierr = rsvdt_span_basis(1, Vm->rows, Vm->cols, X->d, Vm->d, E->d, 0, Vm->cols, 1,
-                                (size_t)Vm->rows * (size_t)Vm->cols * 16, 1e-14);
-    RSVDT_MatDestroy(&X);
-    RSVDT_MatDestroy(&E);
+        ierr = rsvdt_span_basis(1, Vm->rows, Vm->cols, X, Vm->d, E, T, 0, Vm->cols, 1, bytes, 1e-14);
+    rsvdt_hip_sync();
+    if (X) rsvdt_hip_free(X);
+    if (E) rsvdt_hip_free(E);
+    if (T) rsvdt_hip_free(T);
     return ierr;
 }
 
@@ -695,22 +787,25 @@ PetscErrorCode QRDecomposition(Mat Vm)
  *   y1 = RR*Y[:,W1]*dft_m, y2 = RR*Y[:,W2]*dft_m  (W1 = 0..Nw-1, W2 = 1..Nw,
  *   swapped for the adjoint), ph = e^{+-i w m dT},
  *   a  = (ph I - M_tilde)^{-1} V^H (ph y1 - y2),   Y_hat_m = y1 - V a.
- * The two windowed DFTs are one GEMM producing y1 and t = ph*y1 - y2. */
+ * With the window coefficients C1 (y1 = X C1) and Ct (ph y1 - y2 = X Ct),
+ * per test vector: B = (V^H X) Ct, a = solve, Y_hat = X C1 - V a. */
 PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR, DFT_matrices *D)
 {
     (void)LNS;
     PetscInt k = V->RSVD.k, Nw = V->RSVD.Nw, Ns1 = Nw + 1, Nwe = V->RSVD.Nw_eff, N = Yall->rows;
+    if (!TSR->M_tilde_all) return rsvdt_err(RSVDT_ERR_USER, "GalerkinProjAllModes", "M_tilde_all is not set");
     int dir = V->TS.DirAdj ? 1 : 0;
     double dT = V->TS.dt * (double)V->TS.ResRatio;
     double RR = (double)V->TS.ResRatio;
-    Mat Vb;
-    RSVDT_CHK(snapshot_basis(Yall, k, &Vb));
-    /* host: dft (Nw x Nwe) and the window coefficients C[s][p], p < 2*Nwe */
+    const PetscInt kb = tr_group_size(k, N, Ns1);
+    /* host: dft (Nw x Nwe), the window coefficients for every test vector of a
+     * group (C1 then Ct, kb x Ns1 x Nwe each) and the shifts ph (k x Nwe) */
     double *dft = malloc(sizeof(double) * 2 * (size_t)(Nw * Nwe));
-    double *C = calloc((size_t)(Ns1 * 2 * Nwe) * 2, sizeof(double));
+    double *C = calloc((size_t)(2 * kb * Ns1 * Nwe) * 2, sizeof(double));
     double *ph = malloc(sizeof(double) * 2 * (size_t)(k * Nwe));
-    if (!dft || !C || !ph) return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProjAllModes", "out of memory");
-    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_d2h(dft, D->dft->d, sizeof(double) * 2 * (size_t)(Nw * Nwe)));
+    if (!dft || !C || !ph) { free(dft); free(C); free(ph); return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProjAllModes", "out of memory"); }
+    int h = rsvdt_hip_memcpy_d2h(dft, D->dft->d, sizeof(double) * 2 * (size_t)(Nw * Nwe));
+    if (h) { free(dft); free(C); free(ph); return rsvdt_err(h, "GalerkinProjAllModes", "device call failed: %s", rsvdt_hip_last_error()); }
     const PetscInt w1 = dir ? 0 : 1, w2 = dir ? 1 : 0;
     for (PetscInt m = 0; m < Nwe; m++) {
         double ang = (dir ? 1.0 : -1.0) * V->RSVD.w * (double)m * dT;   /* GalerkinProj.c:49-50 */
@@ -719,57 +814,77 @@ PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_r
         for (PetscInt s = 0; s < Ns1; s++) {
             double d1r = 0, d1i = 0, d2r = 0, d2i = 0;
             PetscInt j1 = s - w1, j2 = s - w2;
-            if (j1 >= 0 && j1 < Nw) { d1r = dft[2 * (j1 * Nwe + m)]; d1i = dft[2 * (j1 * Nwe + m) + 1]; }
-            if (j2 >= 0 && j2 < Nw) { d2r = dft[2 * (j2 * Nwe + m)]; d2i = dft[2 * (j2 * Nwe + m) + 1]; }
-            double *c1 = C + 2 * (size_t)(s * 2 * Nwe + m);
-            double *ct = C + 2 * (size_t)(s * 2 * Nwe + Nwe + m);
-            c1[0] = d1r; c1[1] = d1i;
-            ct[0] = pr * d1r - pi * d1i - d2r;
-            ct[1] = pr * d1i + pi * d1r - d2i;
+            if (j1 >= 0 && j1 < Nw) { d1r = RR * dft[2 * (j1 * Nwe + m)]; d1i = RR * dft[2 * (j1 * Nwe + m) + 1]; }
+            if (j2 >= 0 && j2 < Nw) { d2r = RR * dft[2 * (j2 * Nwe + m)]; d2i = RR * dft[2 * (j2 * Nwe + m) + 1]; }
+            for (PetscInt b = 0; b < kb; b++) {
+                double *c1 = C + 2 * (size_t)((b * Ns1 + s) * Nwe + m);
+                double *ct = C + 2 * (size_t)(((kb + b) * Ns1 + s) * Nwe + m);
+                c1[0] = d1r; c1[1] = d1i;
+                ct[0] = pr * d1r - pi * d1i - d2r;
+                ct[1] = pr * d1i + pi * d1r - d2i;
+            }
         }
     }
     free(dft);
-    double *dC, *dph;
-    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_malloc((void **)&dC, sizeof(double) * 2 * (size_t)(Ns1 * 2 * Nwe)));
-    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_malloc((void **)&dph, sizeof(double) * 2 * (size_t)(k * Nwe)));
-    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_h2d(dC, C, sizeof(double) * 2 * (size_t)(Ns1 * 2 * Nwe)));
-    RSVDT_HIPCHK("GalerkinProjAllModes", rsvdt_hip_memcpy_h2d(dph, ph, sizeof(double) * 2 * (size_t)(k * Nwe)));
+    const size_t csz = sizeof(double) * 2 * (size_t)(2 * kb * Ns1 * Nwe);
+    const size_t ybytes = (size_t)kb * (size_t)N * (size_t)Nwe * 16;
+    double *dC = NULL, *dph = NULL, *Ybrm = NULL;
+    PetscErrorCode ierr = 0;
+    if (rsvdt_hip_malloc((void **)&dC, csz) || rsvdt_hip_malloc((void **)&dph, sizeof(double) * 2 * (size_t)(k * Nwe)) ||
+        rsvdt_hip_malloc((void **)&Ybrm, ybytes))
+        ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProjAllModes", "cannot allocate work");
+    if (!ierr && (rsvdt_hip_memcpy_h2d(dC, C, csz) || rsvdt_hip_memcpy_h2d(dph, ph, sizeof(double) * 2 * (size_t)(k * Nwe))))
+        ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProjAllModes", "device call failed: %s", rsvdt_hip_last_error());
     free(C); free(ph);
-    /* y1 (blocks 0..Nwe-1) and t (blocks Nwe..2Nwe-1) */
-    Mat Y1T;
-    RSVDT_CHK(rsvdt_mat_alloc_dense(N, 2 * Nwe * k, k, 0, &Y1T));
-    const int64_t L = N * k;
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_coef_gemm(2 * Nwe, Ns1, L, dC, 2 * Nwe, 1, 0, 1, 2 * Nwe, RR, Yall->d, L, Y1T->d,
-                                                     L, 64, 0.0, g_rsvdt.stream));
-    /* b_ik = V_ik^H t_ik  (k x Ns1 x Nwe) */
-    size_t nb_b = (size_t)k * (size_t)Ns1 * (size_t)Nwe * 16;
+    Mat Yup = NULL;
+    if (!ierr) ierr = rsvdt_mat_alloc_dense(N, Nwe * k, k, 1, &Yup);
+    size_t nb_b = (size_t)kb * (size_t)Ns1 * (size_t)Nwe * 16, nb_w = (size_t)kb * (size_t)Ns1 * (size_t)Ns1 * 16;
     double *B = rsvdt_scratch(SCR_MISC, nb_b);
-    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(k, N, Ns1, Nwe));
-    void *lw = rsvdt_scratch(SCR_G, rsvdt_hip_bshift_solve_work_bytes(k, Ns1, Nwe));
-    int *info = rsvdt_scratch(SCR_INFO, 256);
-    if (!B || !work || !lw || !info) return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
-    const double *T = Y1T->d + 2 * (size_t)Nwe * (size_t)L;
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bgram(k, N, Ns1, Nwe, Vb->d, 1, k, L, T, 1, k, L, B, work, g_rsvdt.stream));
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_zero(info, 256, g_rsvdt.stream));
-    /* a = (ph I - M_tilde)^{-1} b, in place in B */
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bshift_solve(k, Ns1, Nwe, TSR->M_tilde_all->d, dph, B, B, lw, info,
-                                                        g_rsvdt.stream));
-    /* Y_hat(ik) = y1 - V a */
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_zscale((int64_t)k * Ns1 * Nwe, -1.0, B, B, g_rsvdt.stream));
-    Mat Yup;
-    RSVDT_CHK(rsvdt_mat_alloc_dense(N, Nwe * k, k, 1, &Yup));
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_bapply(k, N, Ns1, Nwe, Vb->d, 1, k, L, B, 1.0, Y1T->d, 1, k, L, Yup->d, 1, k,
-                                                  L, g_rsvdt.stream));
+    double *W = rsvdt_scratch(SCR_MISC2, nb_w);
+    int *info = rsvdt_scratch(SCR_GINFO, 256);
+    if (!ierr && (!B || !W || !info))
+        ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
+    rsvdt_stream_t q = g_rsvdt.stream;
+    if (!ierr && rsvdt_hip_zero(info, 256, q)) ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
+    const double *C1 = dC, *Ct = dC + 2 * (size_t)(kb * Ns1 * Nwe);
+    for (PetscInt b0 = 0; b0 < k && !ierr; b0 += kb) {
+        const PetscInt g = kb < k - b0 ? kb : k - b0;
+        ierr = tr_group_build(Yall, k, b0, g, &g_trg);
+        if (ierr) break;
+        const int64_t xs = N * Ns1;
+        /* work sizes follow the group's own batch count */
+        void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
+        void *lw = rsvdt_scratch(SCR_G, rsvdt_hip_bshift_solve_work_bytes(g, Ns1, Nwe));
+        if (!work || !lw) { ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work"); break; }
+#define GCHK(x) do { int _h = (x); if (_h) { ierr = rsvdt_err(_h, "GalerkinProj", "device call failed: %s", rsvdt_hip_last_error()); } } while (0)
+        /* W = V^H X;  B = W Ct = V^H (ph y1 - y2) */
+        GCHK(rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, xs, Ns1, 1, g_trg.X, xs, Ns1, 1, W, work, q));
+        if (!ierr) GCHK(rsvdt_hip_bapply(g, Ns1, Ns1, Nwe, W, Ns1 * Ns1, Ns1, 1, Ct, 0.0, NULL, 0, 0, 0, B, Ns1 * Nwe,
+                                         Nwe, 1, q));
+        /* a = (ph I - M_tilde)^{-1} b, in place in B */
+        if (!ierr) GCHK(rsvdt_hip_bshift_solve(g, Ns1, Nwe, TSR->M_tilde_all->d + 2 * (size_t)b0 * Ns1 * Ns1,
+                                               dph + 2 * (size_t)b0 * Nwe, B, B, lw, info, q));
+        if (!ierr) GCHK(rsvdt_hip_zscale((int64_t)g * Ns1 * Nwe, -1.0, B, B, q));
+        /* Y_hat = X C1 - V a  (per-test-vector rows), then to frequency blocks */
+        if (!ierr) GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.X, xs, Ns1, 1, C1, 0.0, NULL, 0, 0, 0, Ybrm, N * Nwe, Nwe,
+                                         1, q));
+        if (!ierr) GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.V, xs, Ns1, 1, B, 1.0, Ybrm, N * Nwe, Nwe, 1, Ybrm,
+                                         N * Nwe, Nwe, 1, q));
+        if (!ierr) GCHK(rsvdt_hip_transpose_rows(N, Nwe, k, b0, g, Ybrm, Yup->d, 1, q));
+#undef GCHK
+    }
     int hinfo = 0;
-    RSVDT_HIPCHK("GalerkinProj", rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)));
+    if (!ierr && rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)))
+        ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
     if (hinfo) rsvdt_printf("[rsvdt] warning: %d singular Galerkin systems\n", hinfo);
     rsvdt_hip_sync();
-    rsvdt_hip_free(dC); rsvdt_hip_free(dph);
-    RSVDT_MatDestroy(&Y1T);
+    if (dC) rsvdt_hip_free(dC);
+    if (dph) rsvdt_hip_free(dph);
+    if (Ybrm) rsvdt_hip_free(Ybrm);
+    tr_group_free(&g_trg);
+    if (ierr) { RSVDT_MatDestroy(&Yup); return ierr; }
     RSVDT_MatDestroy(&TSR->M_tilde_all);      /* GalerkinProjAllModes.c:41 */
     TSR->Y_hat_up = Yup;                       /* k-major, as the reference leaves it */
-    RSVDT_MatDestroy(&g_trV);
-    g_trV_key = NULL;
     RSVDT_MatDestroy(&Yall);                   /* GalerkinProjAllModes.c:42 */
     return 0;
 }

@@ -104,12 +104,13 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
 PetscErrorCode rsvdt_cholqr(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
                             int64_t xa, double *R_out, double drop_tol);
 /* Orthonormal basis Q of the column span of nbatch tall matrices X (same
- * strides; Q and the work array E each cover `bytes` bytes like X) that may
- * be numerically rank-deficient: CholeskyQR3 with dropping, then the same on
- * the residuals (I - Q Q^H) X of the dropped columns, reorthogonalised, in
- * the dropped columns' places.  Columns dependent to rounding stay zero. */
+ * strides; Q and the work arrays E and T each cover `bytes` bytes like X)
+ * that may be numerically rank-deficient: CholeskyQR3 with dropping, then
+ * the same on the residuals (I - Q Q^H) X of the dropped columns,
+ * reorthogonalised, in the dropped columns' places.  Columns dependent to
+ * rounding stay zero. */
 PetscErrorCode rsvdt_span_basis(PetscInt nbatch, PetscInt nrows, PetscInt n, const double *X, double *Q, double *E,
-                                int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol);
+                                double *T, int64_t xb, int64_t xr, int64_t xa, size_t bytes, double drop_tol);
 /* Left singular vectors (descending) of nbatch tall matrices, in place, and
  * their singular values (device, nbatch x n). */
 PetscErrorCode rsvdt_svd_left(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,

@@ -41,7 +41,10 @@ def test_coef_gemm(L, pkg, P, K, Lc, p0, sgn, cmod, oprec):
 
 
 @pytest.mark.parametrize("nb,nr,na,nc,layout", [(4, 3000, 16, 16, "freq"), (3, 5000, 65, 65, "snap"),
-                                                (2, 100, 5, 33, "freq"), (7, 999, 10, 3, "snap")])
+                                                (2, 100, 5, 33, "freq"), (7, 999, 10, 3, "snap"),
+                                                (3, 4001, 129, 129, "freq"), (2, 777, 129, 64, "freq"),
+                                                (5, 1001, 33, 48, "freq"), (3, 4096, 1, 7, "freq"),
+                                                (64, 2000, 16, 16, "freq"), (2, 3, 20, 20, "freq")])
 def test_bgram_bapply(L, pkg, nb, nr, na, nc, layout):
     print(f"bgram nb={nb} nr={nr} na={na} nc={nc} {layout}", flush=True)
     rng = np.random.default_rng(na * nc)
@@ -221,3 +224,48 @@ def test_horner_stage_rejects_bad_desc(L, pkg):
     d = Desc(0, 16, 1, None, None, None, 1, 64, 0, 1, 0)
     L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
     assert L.rsvdt_hip_horner_stage(ctypes.byref(d), None, None, None, None, 1.0, None, None, None) != 0
+
+
+@pytest.mark.parametrize("nb,nr,n", [(4, 3001, 16), (2, 1500, 64), (3, 777, 40)])
+def test_bapply_in_place(L, pkg, nb, nr, n):
+    """X <- X C in place (the CholeskyQR apply), unit column strides."""
+    rng = np.random.default_rng(nr + n)
+    X = crandn(rng, nb, nr, n)
+    C = crandn(rng, nb, n, n)
+    ref = np.einsum("bra,bac->brc", X, C)
+    dX, dC = Dev(pkg, X), Dev(pkg, C)
+    L.rsvdt_hip_bapply.argtypes = [I64] * 4 + [V, I64, I64, I64, V, ctypes.c_double, V, I64, I64, I64, V, I64, I64,
+                                               I64, V]
+    assert L.rsvdt_hip_bapply(nb, nr, n, n, dX.p, nr * n, n, 1, dC.p, 0.0, None, 0, 0, 0, dX.p, nr * n, n, 1,
+                              None) == 0
+    assert rel(dX.get(np.complex128, X.shape), ref) < 1e-13
+
+
+def test_bapply_in_place_wide_rejected(L, pkg):
+    """More than 64 output columns are split over workgroups: in place is refused."""
+    X = np.zeros((1, 64, 129), complex)
+    dX, dC = Dev(pkg, X), Dev(pkg, np.zeros((1, 129, 129), complex))
+    L.rsvdt_hip_bapply.argtypes = [I64] * 4 + [V, I64, I64, I64, V, ctypes.c_double, V, I64, I64, I64, V, I64, I64,
+                                               I64, V]
+    assert L.rsvdt_hip_bapply(1, 64, 129, 129, dX.p, 64 * 129, 129, 1, dC.p, 0.0, None, 0, 0, 0, dX.p, 64 * 129, 129,
+                              1, None) != 0
+
+
+@pytest.mark.parametrize("n,N,kt,b0,kb", [(129, 3001, 16, 0, 16), (65, 1000, 32, 8, 8), (7, 50, 3, 1, 2),
+                                          (64, 2048, 16, 0, 16)])
+def test_transpose_rows(L, pkg, n, N, kt, b0, kb):
+    """[s][r][b] <-> [j][r][s] for the batch range b0 .. b0+kb-1."""
+    rng = np.random.default_rng(n + kb)
+    Y = crandn(rng, n, N, kt)
+    dY = Dev(pkg, Y)
+    dT = Dev(pkg, nbytes=kb * N * n * 16)
+    L.rsvdt_hip_transpose_rows.argtypes = [I64, I64, I64, I64, I64, V, V, ctypes.c_int, V]
+    assert L.rsvdt_hip_transpose_rows(N, n, kt, b0, kb, dY.p, dT.p, 0, None) == 0
+    T = dT.get(np.complex128, (kb, N, n))
+    assert np.array_equal(T, Y[:, :, b0:b0 + kb].transpose(2, 1, 0))
+    back = np.zeros_like(Y)
+    dB = Dev(pkg, back)
+    assert L.rsvdt_hip_transpose_rows(N, n, kt, b0, kb, dT.p, dB.p, 1, None) == 0
+    B = dB.get(np.complex128, Y.shape)
+    assert np.array_equal(B[:, :, b0:b0 + kb], Y[:, :, b0:b0 + kb])
+    assert not np.any(np.delete(B, np.s_[b0:b0 + kb], axis=2))
