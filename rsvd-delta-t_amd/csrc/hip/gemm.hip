@@ -34,8 +34,9 @@ __global__ __launch_bounds__(256) void k_coef_gemm(
     int sgn, int64_t cmod, double alpha, const double2 *__restrict__ in, int64_t ldi, OV *__restrict__ out,
     int64_t ldo, double beta)
 {
-    extern __shared__ double2 cs[];                 // [Kpad][T*16]
+    extern __shared__ double2 cs[];                 // [Kpad][T*16 + 4]
     constexpr int PW = T * 16;
+    constexpr int PS = PW + 4;                      // row pitch: the 4 rows a K-step reads hit different banks
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t pbase = (int64_t)blockIdx.y * PW;
     const int64_t Kpad = (K + 3) & ~(int64_t)3;
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(256) void k_coef_gemm(
             if (col < 0 || col >= cmod) { col %= cmod; if (col < 0) col += cmod; }
             v = coef[q * cq + col * cp];
         }
-        cs[e] = v;
+        cs[q * PS + pp] = v;
     }
     __syncthreads();
     const int kq = lane >> 4, cl = lane & 15;
@@ -60,18 +61,29 @@ __global__ __launch_bounds__(256) void k_coef_gemm(
         for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
         const double2 *src = in + l;
         const int nks = (int)(Kpad >> 2);
+        // two K-steps of the streamed operand in flight ahead of the MFMAs
+        double2 x0 = (lok && kq < K) ? src[(int64_t)kq * ldi] : make_double2(0, 0);
+        double2 x1 = (lok && 4 + kq < K) ? src[(int64_t)(4 + kq) * ldi] : make_double2(0, 0);
         for (int ks = 0; ks < nks; ++ks) {
             const int64_t q0 = (int64_t)ks * 4;
-            const int64_t q = q0 + kq;
-            const double2 x = (lok && q < K) ? src[q * ldi] : make_double2(0, 0);
-            const double2 *crow = cs + (q0 + kq) * PW + cl;
+            const int64_t qn = q0 + 8 + kq;
+            const double2 x = x0;
+            x0 = x1;
+            x1 = (lok && qn < K) ? src[qn * ldi] : make_double2(0, 0);
+            const double2 *crow = cs + (q0 + kq) * PS + cl;
+            double2 c[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) c[t] = crow[t * 16];
+            // 2T independent accumulators between dependent MFMAs
 #pragma unroll
             for (int t = 0; t < T; ++t) {
-                const double2 c = crow[t * 16];
-                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.x, x.x, ar[t], 0, 0, 0);
-                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.x, x.y, ai[t], 0, 0, 0);
-                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-c.y, x.y, ar[t], 0, 0, 0);
-                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c.y, x.x, ai[t], 0, 0, 0);
+                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.x, ar[t], 0, 0, 0);
+                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.y, ai[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-c[t].y, x.y, ar[t], 0, 0, 0);
+                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].y, x.x, ai[t], 0, 0, 0);
             }
         }
         if (lok) {
@@ -99,7 +111,7 @@ static int launch_coef(int64_t P, int64_t K, int64_t L, const double *coef, int6
                        int out_prec, double beta, hipStream_t s)
 {
     const int64_t Kpad = (K + 3) & ~(int64_t)3;
-    const size_t lds = (size_t)Kpad * T * 16 * 16;
+    const size_t lds = (size_t)Kpad * (T * 16 + 4) * 16;
     const int64_t ntile = (L + 63) / 64;
     // enough workgroups to fill 256 CUs several times; each visits ntile/gx tiles
     const int64_t per_cu = lds > 80 * 1024 ? 1 : (lds > 40 * 1024 ? 2 : 4);
@@ -435,16 +447,18 @@ extern "C" int rsvdt_hip_coef_gemm(int64_t P, int64_t K, int64_t L, const double
     }
     hipStream_t st = as_stream(s);
     const int64_t Kpad = (K + 3) & ~(int64_t)3;
-    if (Kpad * 16 * 16 * 2 > 150 * 1024) {
-        set_error_msg("rsvdt_hip_coef_gemm: K too large for the LDS coefficient block (K <= 296)");
+    // LDS of the coefficient block for a row tile of 16 T rows (padded pitch)
+    auto lds_of = [Kpad](int T) { return Kpad * (16 * T + 4) * 16; };
+    if (lds_of(1) > 150 * 1024) {
+        set_error_msg("rsvdt_hip_coef_gemm: K too large for the LDS coefficient block (K <= 480)");
         return RSVDT_HIP_ERR_ARG;
     }
     // widest row tile whose coefficient block fits in 144 KB of LDS
-    if (P > 64 && Kpad * 8 * 16 * 16 <= 144 * 1024)
+    if (P > 64 && lds_of(8) <= 144 * 1024)
         return launch_coef<8>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
-    if (P > 32 && Kpad * 4 * 16 * 16 <= 144 * 1024)
+    if (P > 32 && lds_of(4) <= 144 * 1024)
         return launch_coef<4>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
-    if (P > 16)
+    if (P > 16 && lds_of(2) <= 144 * 1024)
         return launch_coef<2>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
     return launch_coef<1>(P, K, L, coef, cq, cp, p0, sgn, cmod, alpha, in, ldi, out, ldo, out_prec, beta, st);
 }

@@ -22,8 +22,11 @@
 
 namespace rsvdt {
 
-template <typename T, bool AREAL> struct aval { using type = typename vec2<T>::type; };
-template <typename T> struct aval<T, true> { using type = T; };
+// Matrix values are FP64 for both state precisions (real or complex); the
+// SpMM accumulates in FP64 and only the stored state vectors are FP32 in the
+// FP32 form (rounding A itself to FP32 would perturb the operator).
+template <typename T, bool AREAL> struct aval { using type = double2; };
+template <typename T> struct aval<T, true> { using type = double; };
 
 __device__ __forceinline__ double shfl_val(double v, int src, int w) { return __shfl(v, src, w); }
 __device__ __forceinline__ float shfl_val(float v, int src, int w) { return __shfl(v, src, w); }
@@ -64,7 +67,8 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 v, bool nt) {
 template <typename T, bool AREAL, int LPR, int CPL, int NB>
 __device__ __forceinline__ void row_gather(int e0, int e1, int lane, int k, const int32_t *__restrict__ colidx,
                                            const typename aval<T, AREAL>::type *__restrict__ vals,
-                                           const typename vec2<T>::type *__restrict__ ub, T (&sr)[CPL], T (&si)[CPL])
+                                           const typename vec2<T>::type *__restrict__ ub, double (&sr)[CPL],
+                                           double (&si)[CPL])
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
@@ -88,26 +92,29 @@ __device__ __forceinline__ void row_gather(int e0, int e1, int lane, int k, cons
             cs[u] = __shfl(mc[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
             as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
         }
+        // unconditional gathers, all in flight together: past the row's end
+        // cs[u] is 0 (a valid row) and the product is skipped below; lanes
+        // beyond k read column k-1 and their sums are never stored
         V xs[NB][CPL];
 #pragma unroll
         for (int u = 0; u < NB; ++u)
 #pragma unroll
             for (int t = 0; t < CPL; ++t) {
                 const int j = lane + t * LPR;
-                xs[u][t].x = 0; xs[u][t].y = 0;
-                if (u < cnt && j < k) xs[u][t] = ub[(int64_t)cs[u] * k + j];
+                xs[u][t] = ub[(int64_t)cs[u] * k + (j < k ? j : k - 1)];
             }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             if (u < cnt) {
 #pragma unroll
                 for (int t = 0; t < CPL; ++t) {
+                    const double xr = (double)xs[u][t].x, xi = (double)xs[u][t].y;
                     if constexpr (AREAL) {
-                        sr[t] += as[u] * xs[u][t].x;
-                        si[t] += as[u] * xs[u][t].y;
+                        sr[t] += as[u] * xr;
+                        si[t] += as[u] * xi;
                     } else {
-                        sr[t] += as[u].x * xs[u][t].x - as[u].y * xs[u][t].y;
-                        si[t] += as[u].x * xs[u][t].y + as[u].y * xs[u][t].x;
+                        sr[t] += as[u].x * xr - as[u].y * xi;
+                        si[t] += as[u].x * xi + as[u].y * xr;
                     }
                 }
             }
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
     const typename vec2<T>::type *__restrict__ uin, const typename vec2<T>::type *yv,
     const typename vec2<T>::type *__restrict__ F, typename vec2<T>::type *__restrict__ acc,
-    typename vec2<T>::type *out, T cnext, T cacc, double2 *__restrict__ snap)
+    typename vec2<T>::type *out, double cnext, double cacc, double2 *__restrict__ snap)
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
@@ -155,9 +162,9 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
             if (stage != 1) ar[t] = ld_stream(acc + idx, NT);
         }
     }
-    T sr[CPL], si[CPL];
+    double sr[CPL], si[CPL];
 #pragma unroll
-    for (int t = 0; t < CPL; ++t) { sr[t] = fr[t].x; si[t] = fr[t].y; }
+    for (int t = 0; t < CPL; ++t) { sr[t] = (double)fr[t].x; si[t] = (double)fr[t].y; }
 
     row_gather<T, AREAL, LPR, CPL, NB>(e0, e1, lane, k, colidx, vals, ub, sr, si);
 #pragma unroll
@@ -167,22 +174,22 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
         const int64_t idx = boff + (int64_t)row * k + j;
         if (stage == 4) {
             V yn;
-            yn.x = ar[t].x + cacc * sr[t];
-            yn.y = ar[t].y + cacc * si[t];
+            yn.x = (T)((double)ar[t].x + cacc * sr[t]);
+            yn.y = (T)((double)ar[t].y + cacc * si[t]);
             out[idx] = yn;
             if (snap != nullptr) st_stream(snap + idx, make_double2((double)yn.x, (double)yn.y), NT);
         } else {
             V u;
-            u.x = yr[t].x + cnext * sr[t];
-            u.y = yr[t].y + cnext * si[t];
+            u.x = (T)((double)yr[t].x + cnext * sr[t]);
+            u.y = (T)((double)yr[t].y + cnext * si[t]);
             out[idx] = u;                     // gathered by the next stage: keep it cacheable
             V a;
             if (stage == 1) {
-                a.x = yr[t].x + cacc * sr[t];
-                a.y = yr[t].y + cacc * si[t];
+                a.x = (T)((double)yr[t].x + cacc * sr[t]);
+                a.y = (T)((double)yr[t].y + cacc * si[t]);
             } else {
-                a.x = ar[t].x + cacc * sr[t];
-                a.y = ar[t].y + cacc * si[t];
+                a.x = (T)((double)ar[t].x + cacc * sr[t]);
+                a.y = (T)((double)ar[t].y + cacc * si[t]);
             }
             st_stream(acc + idx, a, NT);
         }
@@ -200,7 +207,7 @@ static int launch_rk4(const rsvdt_spmm_desc *d, int stage, const void *uin, cons
     const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
     hipLaunchKernelGGL((k_rk4_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s,
                        (int)d->nrows, (int)d->k, stage, remap, d->nrows * d->k, d->rowptr, d->col, d->val,
-                       (const V *)uin, (const V *)y, (const V *)F, (V *)acc, (V *)out, (T)cn, (T)ca,
+                       (const V *)uin, (const V *)y, (const V *)F, (V *)acc, (V *)out, cn, ca,
                        (double2 *)snap);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_rk4_stage");
     return RSVDT_HIP_OK;
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(256) void k_horner_stage(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
     const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
     const double2 *__restrict__ ys, const typename vec2<T>::type *__restrict__ g,
-    typename vec2<T>::type *out, T c, double2 *__restrict__ snap)
+    typename vec2<T>::type *out, double c, double2 *__restrict__ snap)
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
@@ -268,27 +275,22 @@ __global__ __launch_bounds__(256) void k_horner_stage(
     const int e0 = rowptr[row], e1 = rowptr[row + 1];
 
     V yr[CPL], gr[CPL];
+    // when y is also the gathered vector (first stage) its lines are reused
+    // by the gathers: no nontemporal hint on it then
+    const bool nty = NT && (const void *)yv != (const void *)vin;
 #pragma unroll
     for (int t = 0; t < CPL; ++t) {
         const int j = lane + t * LPR;
         const int64_t idx = boff + (int64_t)row * k + j;
         yr[t].x = 0; yr[t].y = 0; gr[t] = yr[t];
         if (j < k) {
-            yr[t] = ld_stream(yv + idx, NT);
+            yr[t] = ld_stream(yv + idx, nty);
             if (g != nullptr) gr[t] = ld_stream(g + idx, NT);
         }
     }
-    if (ys != nullptr) {
-        const double2 sc = ys[blockIdx.y];
-        const T cr = (T)sc.x, ci = (T)sc.y;
-#pragma unroll
-        for (int t = 0; t < CPL; ++t) {
-            const V v = yr[t];
-            yr[t].x = cr * v.x - ci * v.y;
-            yr[t].y = cr * v.y + ci * v.x;
-        }
-    }
-    T sr[CPL], si[CPL];
+    double ysr = 1.0, ysi = 0.0;
+    if (ys != nullptr) { ysr = ys[blockIdx.y].x; ysi = ys[blockIdx.y].y; }
+    double sr[CPL], si[CPL];
 #pragma unroll
     for (int t = 0; t < CPL; ++t) { sr[t] = 0; si[t] = 0; }
     row_gather<T, AREAL, LPR, CPL, NB>(e0, e1, lane, k, colidx, vals, vin + boff, sr, si);
@@ -297,10 +299,17 @@ __global__ __launch_bounds__(256) void k_horner_stage(
         const int j = lane + t * LPR;
         if (j >= k) continue;
         const int64_t idx = boff + (int64_t)row * k + j;
+        double yx = (double)yr[t].x, yy = (double)yr[t].y;
+        if (ys != nullptr) {
+            const double tx = ysr * yx - ysi * yy;
+            yy = ysr * yy + ysi * yx;
+            yx = tx;
+        }
+        double ox = yx + c * sr[t], oy = yy + c * si[t];
+        if (g != nullptr) { ox += (double)gr[t].x; oy += (double)gr[t].y; }
         V o;
-        o.x = yr[t].x + c * sr[t];
-        o.y = yr[t].y + c * si[t];
-        if (g != nullptr) { o.x += gr[t].x; o.y += gr[t].y; }
+        o.x = (T)ox;
+        o.y = (T)oy;
         out[idx] = o;                         // gathered by the next stage: keep it cacheable
         if (snap != nullptr) st_stream(snap + idx, make_double2((double)o.x, (double)o.y), NT);
     }
@@ -317,7 +326,7 @@ static int launch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *
     const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
     hipLaunchKernelGGL((k_horner_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows,
                        (int)d->k, remap, d->nrows * d->k, d->rowptr, d->col, d->val, (const V *)vin, (const V *)y,
-                       (const double2 *)ys, (const V *)g, (V *)out, (T)c, (double2 *)snap);
+                       (const double2 *)ys, (const V *)g, (V *)out, c, (double2 *)snap);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
     return RSVDT_HIP_OK;
 }

@@ -68,7 +68,7 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->nb = nb;
     d->rowptr = dir ? A->d_rowptr : A->d_rowptrH;
     d->col = dir ? A->d_col : A->d_colH;
-    d->val = dir ? (prec == 64 ? A->d_val64 : A->d_val32) : (prec == 64 ? A->d_valH64 : A->d_valH32);
+    d->val = dir ? A->d_val64 : A->d_valH64;      /* FP64 values for both state precisions */
     d->a_real = A->is_real;
     d->prec = prec;
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;

@@ -27,14 +27,15 @@ struct _p_RSVDT_Mat {
     PetscInt kb;
     int order;
     double *d;
-    /* AIJ (CSR): host copy (int64, complex) and device copies (int32 indices)
-     * of the matrix and of its conjugate transpose, FP64 and FP32 values. */
+    /* AIJ (CSR): host copy (int64, complex) and device copies (int32 indices,
+     * FP64 values, real when is_real) of the matrix and of its conjugate
+     * transpose. */
     PetscInt nnz;
     int is_real;             /* ||Im||_F < 1e-12: values kept real on the device */
     int64_t *h_rowptr, *h_col;
     double *h_val;           /* 2*nnz */
     int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
-    void *d_val64, *d_val32, *d_valH64, *d_valH32;
+    void *d_val64, *d_valH64;
     /* autotuned SpMM launch shape per [stepping form][precision 64/32]
      * [direct/adjoint]: lanes per row, kernel variant, XCD remap; tuned_ok
      * marks a tuned entry */

@@ -426,8 +426,8 @@ PetscErrorCode RSVDT_MatDestroy(Mat *pm)
     if (g_rsvdt.device >= 0 || m->d || m->d_rowptr) rsvdt_hip_free(m->d);
     rsvdt_hip_free(m->d_rowptr); rsvdt_hip_free(m->d_col);
     rsvdt_hip_free(m->d_rowptrH); rsvdt_hip_free(m->d_colH);
-    rsvdt_hip_free(m->d_val64); rsvdt_hip_free(m->d_val32);
-    rsvdt_hip_free(m->d_valH64); rsvdt_hip_free(m->d_valH32);
+    rsvdt_hip_free(m->d_val64);
+    rsvdt_hip_free(m->d_valH64);
     free(m->h_rowptr); free(m->h_col); free(m->h_val);
     free(m);
     *pm = NULL;
@@ -480,7 +480,7 @@ PetscErrorCode rsvdt_aij_real_check(Mat m)
 }
 
 static PetscErrorCode upload_csr(PetscInt rows, const int64_t *rp, const int64_t *ci, const double *v, PetscInt nnz,
-                                 int is_real, int32_t **d_rp, int32_t **d_ci, void **d64, void **d32)
+                                 int is_real, int32_t **d_rp, int32_t **d_ci, void **d64)
 {
     if (nnz > 0x7fffffffLL || rows > 0x7fffffffLL)
         return rsvdt_err(RSVDT_ERR_USER, "MatLoad", "operator too large for 32-bit device indices");
@@ -488,23 +488,20 @@ static PetscErrorCode upload_csr(PetscInt rows, const int64_t *rp, const int64_t
     int32_t *hci = malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
     size_t nv = (size_t)(nnz ? nnz : 1) * (is_real ? 1 : 2);
     double *h64 = malloc(sizeof(double) * nv);
-    float *h32 = malloc(sizeof(float) * nv);
-    if (!hrp || !hci || !h64 || !h32) return rsvdt_err(RSVDT_ERR_MEM, "MatLoad", "out of memory");
+    if (!hrp || !hci || !h64) { free(hrp); free(hci); free(h64); return rsvdt_err(RSVDT_ERR_MEM, "MatLoad", "out of memory"); }
     for (PetscInt r = 0; r <= rows; r++) hrp[r] = (int32_t)rp[r];
     for (PetscInt e = 0; e < nnz; e++) {
         hci[e] = (int32_t)ci[e];
-        if (is_real) { h64[e] = v[2 * e]; h32[e] = (float)v[2 * e]; }
-        else { h64[2 * e] = v[2 * e]; h64[2 * e + 1] = v[2 * e + 1]; h32[2 * e] = (float)v[2 * e]; h32[2 * e + 1] = (float)v[2 * e + 1]; }
+        if (is_real) h64[e] = v[2 * e];
+        else { h64[2 * e] = v[2 * e]; h64[2 * e + 1] = v[2 * e + 1]; }
     }
     int rc = rsvdt_hip_malloc((void **)d_rp, sizeof(int32_t) * (size_t)(rows + 1));
     if (!rc) rc = rsvdt_hip_malloc((void **)d_ci, sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
     if (!rc) rc = rsvdt_hip_malloc(d64, sizeof(double) * nv);
-    if (!rc) rc = rsvdt_hip_malloc(d32, sizeof(float) * nv);
     if (!rc) rc = rsvdt_hip_memcpy_h2d(*d_rp, hrp, sizeof(int32_t) * (size_t)(rows + 1));
     if (!rc) rc = rsvdt_hip_memcpy_h2d(*d_ci, hci, sizeof(int32_t) * (size_t)nnz);
     if (!rc) rc = rsvdt_hip_memcpy_h2d(*d64, h64, sizeof(double) * nv);
-    if (!rc) rc = rsvdt_hip_memcpy_h2d(*d32, h32, sizeof(float) * nv);
-    free(hrp); free(hci); free(h64); free(h32);
+    free(hrp); free(hci); free(h64);
     if (rc) return rsvdt_err(rc, "MatLoad", "%s", rsvdt_hip_last_error());
     return 0;
 }
@@ -519,7 +516,7 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     RSVDT_CHK(rsvdt_ensure_device());
     RSVDT_CHK(rsvdt_aij_real_check(m));
     RSVDT_CHK(upload_csr(m->rows, m->h_rowptr, m->h_col, m->h_val, m->nnz, m->is_real, &m->d_rowptr, &m->d_col,
-                         &m->d_val64, &m->d_val32));
+                         &m->d_val64));
     PetscInt nr = m->cols, nnz = m->nnz;
     int64_t *rp = calloc((size_t)nr + 1, sizeof(int64_t));
     int64_t *ci = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
@@ -537,8 +534,7 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
             v[2 * p] = m->h_val[2 * e];
             v[2 * p + 1] = -m->h_val[2 * e + 1];
         }
-    PetscErrorCode ierr = upload_csr(nr, rp, ci, v, nnz, m->is_real, &m->d_rowptrH, &m->d_colH, &m->d_valH64,
-                                     &m->d_valH32);
+    PetscErrorCode ierr = upload_csr(nr, rp, ci, v, nnz, m->is_real, &m->d_rowptrH, &m->d_colH, &m->d_valH64);
     free(rp); free(ci); free(v); free(fill);
     memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
     return ierr;

@@ -160,8 +160,7 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
     y = crandn(rng, nb, N, k); F = crandn(rng, 3, nb, N, k)
     ref = np.stack([rk4_ref(A, y[b], F[0, b], F[1, b], F[2, b], h) for b in range(nb)])
     cdt = np.complex128 if prec == 64 else np.complex64
-    rdt = np.float64 if prec == 64 else np.float32
-    vals = A.data.real.astype(rdt) if real else A.data.astype(cdt)
+    vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)   # FP64 values for both state precisions
     drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
     dy = Dev(pkg, y.astype(cdt)); dF = [Dev(pkg, F[i].astype(cdt)) for i in range(3)]
     du1, du2, dacc = (Dev(pkg, nbytes=y.astype(cdt).nbytes) for _ in range(3))
@@ -202,8 +201,7 @@ def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced)
     ys = crandn(rng, nb) if scaled else np.ones(nb, complex)
     ref = np.stack([ys[b] * y[b] + c * (A @ vin[b]) + (g[b] if forced else 0) for b in range(nb)])
     cdt = np.complex128 if prec == 64 else np.complex64
-    rdt = np.float64 if prec == 64 else np.float32
-    vals = A.data.real.astype(rdt) if real else A.data.astype(cdt)
+    vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)   # FP64 values for both state precisions
     drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
     dvin, dy, dg = Dev(pkg, vin.astype(cdt)), Dev(pkg, y.astype(cdt)), Dev(pkg, g.astype(cdt))
     dys = Dev(pkg, ys.astype(np.complex128))
