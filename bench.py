@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-overlap", action="store_true", help="forcing GEMM on the RK4 stream")
+    ap.add_argument("--workload", default="flow2d", choices=("flow2d", "flow3d"),
+                    help="flow2d: the metric's workload; flow3d: the 3D operator (k=32, Nw=64)")
     ap.add_argument("--rk4-form", type=int, default=0, choices=(0, 1),
                     help="0: Horner stages with the folded forcing (default); 1: the reference's stage order")
     ap.add_argument("--calib-copy", action="store_true",
@@ -114,10 +116,17 @@ def main():
         calib_copy(pkg)
         return
     import workloads
-    A = workloads.flow2d()
+    if args.workload == "flow2d":
+        A = workloads.flow2d()
+        k, Nw, w = 16, 128, 2 * np.pi / 20
+        dt = workloads.flow2d_dt()
+        wl = "flow2d N=393216 (3 fields, 512x256 periodic, 11 nnz/row)"
+    else:
+        A = workloads.flow3d()
+        k, Nw, w = 32, 64, 2 * np.pi / 20
+        dt = workloads.flow3d_dt()
+        wl = "flow3d N=1966080 (5 fields, 128x64x48, periodic x/z, Dirichlet y, 11 nnz/row)"
     N = A.shape[0]
-    k, Nw, w = 16, 128, 2 * np.pi / 20
-    dt = workloads.flow2d_dt()
     T = 2 * np.pi / w
     TL = T / 2
     s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, precision=args.precision,
@@ -129,14 +138,16 @@ def main():
     S = st["chunk_steps"]
     form = st["rk4_form"]
     esz = 16 if args.precision == 64 else 8
-    vsz = 8 if args.precision == 64 else 4
     V = N * k * esz
-    S_A = A.nnz * (vsz + 4) + 4 * (N + 1)
+    # the matrix as the kernels read it (FP64 real values, int32 columns), and
+    # as the reference formulation of the same precision would store it
+    S_A = A.nnz * (8 + 4) + 4 * (N + 1)
+    S_A_ref = A.nnz * ((8 if args.precision == 64 else 4) + 4) + 4 * (N + 1)
     Nwe = s.Nw_eff
     # the metric's byte count: the reference formulation of one step (4 SpMM
     # with the stage AXPYs, 20 block reads/writes, plus the forcing samples of
     # a 64-sample chunk), whatever form the engine runs
-    bytes_step_ref = 4 * S_A + 20 * V + (2 * Nwe / 64 + 2) * V
+    bytes_step_ref = 4 * S_A_ref + 20 * V + (2 * Nwe / 64 + 2) * V
     # what the form that ran needs per step
     if form == 0:
         bytes_step_alg = 4 * S_A + 12 * V + (Nwe / S + 1) * V
@@ -162,9 +173,8 @@ def main():
         "vs_baseline": None,
         "dtype": "c128" if args.precision == 64 else "c64",
         "data": "synthetic",
-        "config": {"workload": "flow2d N=393216 (3 fields, 512x256 periodic, 11 nnz/row) RK4 time stepping, "
-                               "k=16, Nw=128 (Nw_eff=64), transient removal, FP64" if args.precision == 64 else
-                               "flow2d N=393216 RK4 time stepping, k=16, Nw=128, FP32 state",
+        "config": {"workload": f"{wl} RK4 time stepping, k={k}, Nw={Nw} (Nw_eff={Nwe}), transient removal, "
+                               + ("FP64" if args.precision == 64 else "FP32 state"),
                    "N": N, "k": k, "Nw": Nw, "dt": dt, "chunk_steps": S, "precision": args.precision,
                    "rk4_form": "horner+folded forcing" if form == 0 else "reference stage order",
                    "bytes_per_step_reference_form": int(bytes_step_ref),
@@ -179,7 +189,7 @@ def main():
         "breakdown_ms_per_step": {"rk4": round(rk_ms / args.steps, 5), "forcing": round(fr_ms / args.steps, 5)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(A, k, Nw, w, TL, dt, bytes_step)
+        out["cpu_baseline"] = cpu_baseline(A, k, Nw, w, TL, dt, bytes_step_ref)
     if args.full:
         rng = None
         res = {}

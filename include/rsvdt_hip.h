@@ -83,8 +83,9 @@ typedef struct {
     int lanes;            /* 0 = default choice */
     int xcd_remap;        /* 1 = give each XCD a contiguous row range */
     int variant;          /* 0: 16-nonzero gather batches; 1: + nontemporal row-local
-                             streams; 2: 8-nonzero batches; 3: 8 + nontemporal
-                             (variants apply when k <= lanes) */
+                             streams; 2: 8-nonzero batches; 3: 8 + nontemporal; 4: 12-nonzero
+                             batches; 5: 12 + nontemporal (variants apply when k <= lanes;
+                             4 and 5 only to the Horner stage with lanes >= 16) */
 } rsvdt_spmm_desc;
 
 int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,

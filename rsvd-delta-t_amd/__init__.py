@@ -165,6 +165,8 @@ def lib():
     L.SetupTimeFreqGrid.argtypes = [P(RSVDt_vars), P(TransRun_vars)]
     L.CreateDFTiDFTMats.argtypes = [P(RSVDt_vars), P(DFT_matrices), P(LNS_vars)]
     L.QRAllFreqs.argtypes = [P(RSVD_matrices), P(RSVDt_vars)]
+    L.TSExpADeltaT.argtypes = [P(LNS_vars), P(RSVDt_vars), _vp]
+    L.QRDecomposition.argtypes = [_vp]
     L.SVDAllFreqsBeforeAdjoint.argtypes = [P(RSVD_matrices), P(RSVDt_vars)]
     L.StoreU.argtypes = [P(RSVD_matrices), P(Resolvent_matrices)]
     L.SVDAllFreqs.argtypes = [P(RSVD_matrices), P(RSVDt_vars), P(Weight_matrices), P(Resolvent_matrices)]

@@ -53,54 +53,70 @@ __global__ __launch_bounds__(256) void k_coef_gemm(
     __syncthreads();
     const int kq = lane >> 4, cl = lane & 15;
     const int64_t ntile = (L + 63) / 64;
-    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        const int64_t l = tile * 64 + wv * 16 + cl;
-        const bool lok = l < L;
-        dvec4 ar[T], ai[T];
+    // The workgroup's (tile, K-step) sequence is walked as one stream with the
+    // streamed operand loaded PF steps ahead, across tile boundaries too.
+    constexpr int PF = 4;
+    const int nks = (int)(Kpad >> 2);
+    const int64_t nmine = blockIdx.x < ntile ? (ntile - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+    const int64_t nsteps = nmine * nks;
+    auto load_step = [&](int64_t gs) -> double2 {
+        if (gs >= nsteps) return make_double2(0, 0);
+        const int64_t it = gs / nks, q = (gs - it * nks) * 4 + kq;
+        const int64_t l = (blockIdx.x + it * gridDim.x) * 64 + wv * 16 + cl;
+        return (l < L && q < K) ? in[q * ldi + l] : make_double2(0, 0);
+    };
+    double2 ring[PF];
 #pragma unroll
-        for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
-        const double2 *src = in + l;
-        const int nks = (int)(Kpad >> 2);
-        // two K-steps of the streamed operand in flight ahead of the MFMAs
-        double2 x0 = (lok && kq < K) ? src[(int64_t)kq * ldi] : make_double2(0, 0);
-        double2 x1 = (lok && 4 + kq < K) ? src[(int64_t)(4 + kq) * ldi] : make_double2(0, 0);
-        for (int ks = 0; ks < nks; ++ks) {
-            const int64_t q0 = (int64_t)ks * 4;
-            const int64_t qn = q0 + 8 + kq;
-            const double2 x = x0;
-            x0 = x1;
-            x1 = (lok && qn < K) ? src[qn * ldi] : make_double2(0, 0);
-            const double2 *crow = cs + (q0 + kq) * PS + cl;
-            double2 c[T];
+    for (int p = 0; p < PF; ++p) ring[p] = load_step(p);
+    dvec4 ar[T], ai[T];
 #pragma unroll
-            for (int t = 0; t < T; ++t) c[t] = crow[t * 16];
-            // 2T independent accumulators between dependent MFMAs
+    for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
+    for (int64_t g0 = 0; g0 < nsteps; g0 += PF) {
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.x, ar[t], 0, 0, 0);
-                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.y, ai[t], 0, 0, 0);
-            }
+        for (int p = 0; p < PF; ++p) {
+            const int64_t gs = g0 + p;
+            if (gs < nsteps) {
+                const double2 x = ring[p];
+                ring[p] = load_step(gs + PF);
+                const int64_t it = gs / nks;
+                const int ks = (int)(gs - it * nks);
+                const double2 *crow = cs + ((int64_t)ks * 4 + kq) * PS + cl;
+                double2 c[T];
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-c[t].y, x.y, ar[t], 0, 0, 0);
-                ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].y, x.x, ai[t], 0, 0, 0);
-            }
-        }
-        if (lok) {
+                for (int t = 0; t < T; ++t) c[t] = crow[t * 16];
+                // 2T independent accumulators between dependent MFMAs
 #pragma unroll
-            for (int t = 0; t < T; ++t)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int64_t p = pbase + t * 16 + 4 * j + kq;
-                    if (p < P) {
-                        OV *o = out + p * ldo + l;
-                        double orr = alpha * ar[t][j], oi = alpha * ai[t][j];
-                        if (beta != 0.0) { OV v = *o; orr += beta * (double)v.x; oi += beta * (double)v.y; }
-                        OV w;
-                        w.x = orr; w.y = oi;
-                        *o = w;
-                    }
+                for (int t = 0; t < T; ++t) {
+                    ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.x, ar[t], 0, 0, 0);
+                    ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].x, x.y, ai[t], 0, 0, 0);
                 }
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    ar[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-c[t].y, x.y, ar[t], 0, 0, 0);
+                    ai[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(c[t].y, x.x, ai[t], 0, 0, 0);
+                }
+                if (ks == nks - 1) {                  // tile done: store and restart the sums
+                    const int64_t l = (blockIdx.x + it * gridDim.x) * 64 + wv * 16 + cl;
+                    if (l < L) {
+#pragma unroll
+                        for (int t = 0; t < T; ++t)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int64_t pp = pbase + t * 16 + 4 * j + kq;
+                                if (pp < P) {
+                                    OV *o = out + pp * ldo + l;
+                                    double orr = alpha * ar[t][j], oi = alpha * ai[t][j];
+                                    if (beta != 0.0) { OV v = *o; orr += beta * (double)v.x; oi += beta * (double)v.y; }
+                                    OV w;
+                                    w.x = orr; w.y = oi;
+                                    *o = w;
+                                }
+                            }
+                    }
+#pragma unroll
+                    for (int t = 0; t < T; ++t) { ar[t] = (dvec4){0, 0, 0, 0}; ai[t] = (dvec4){0, 0, 0, 0}; }
+                }
+            }
         }
     }
 }

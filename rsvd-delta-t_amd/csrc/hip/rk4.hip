@@ -345,6 +345,10 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
     const int var = d->variant;
 #define RSVDT_HCASE(L, C)                                                                                         \
     if (lpr == L && cpl <= C) {                                                                                   \
+        if constexpr (C == 1 && L >= 16) {                                                                        \
+            if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, s);     \
+            if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, s);      \
+        }                                                                                                         \
         if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, s); \
         if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, s); \
         if (C == 1 && var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, s);  \

@@ -184,7 +184,8 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
     (1, False, 64, 1, 0, 0, False, True), (16, True, 64, 1, 0, 3, False, True), (16, True, 64, 1, 16, 1, True, False),
     (10, False, 64, 3, 0, 0, True, True), (16, False, 64, 2, 8, 0, False, False), (32, True, 64, 1, 0, 2, False, True),
     (40, False, 64, 1, 0, 0, True, False), (16, True, 32, 1, 0, 3, False, True), (16, True, 32, 1, 8, 0, False, True),
-    (10, False, 32, 2, 4, 0, True, True)])
+    (10, False, 32, 2, 4, 0, True, True), (16, True, 64, 1, 16, 4, False, True), (16, False, 64, 2, 16, 5, True, True),
+    (32, True, 32, 1, 32, 4, False, True), (12, True, 64, 1, 16, 5, False, False)])
 def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced):
     """out_b = ys_b y_b + c A vin_b + g_b, with the snapshot copy, against numpy."""
     print(f"horner k={k} real={real} prec={prec} nb={nb} lanes={lanes} variant={variant} scaled={scaled} "
