@@ -210,6 +210,33 @@ int rsvdt_hip_transpose_rows(int64_t nrows, int64_t n, int64_t kt, int64_t b0, i
 /* sum of |x|^2 over n complex entries into *out (device double) */
 int rsvdt_hip_znorm2(int64_t n, const double *x, double *out_dev, void *work, rsvdt_stream_t s);
 
+/* ---- learned sketch (coordinate MLP, BF16 MFMA) ------------------------ *
+ * No counterpart in the reference: the network that proposes part of the
+ * initial forcing sketch (the random columns of SourceCode/CreateRandomMat.c:39
+ * otherwise).  All arrays FP32 unless noted.                              */
+/* features (n x 64) of points q (n x 4 FP64: x, w, mu0, Re U):
+ * [x/xs, w/ws, mu0, U, sin(kj x), cos(kj x) for kj = kmax (j+1)/28, j < 28, 0 x 4] */
+int rsvdt_hip_mlp_features(int64_t n, const double *q, double xs, double ws, double kmax, float *out,
+                           rsvdt_stream_t s);
+/* C(m,n) = sum_k A(m,k) B(k,n) with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
+ * BF16 operands (rounded to nearest even), FP32 accumulation, C row stride ldc.
+ * epilogue 0: C = acc; 1: z = acc + bias[n], Z(m,n) = z, C = z sigmoid(z);
+ * 2: C = acc + bias[n]; 3: C = acc * silu'(Z(m,n)).  Z row stride ldz. */
+int rsvdt_hip_gemm_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam, int64_t sak, const float *B,
+                        int64_t sbk, int64_t sbn, float *C, int64_t ldc, int epilogue, const float *bias, float *Z,
+                        int64_t ldz, rsvdt_stream_t s);
+/* out[n] = sum_{m<M} A[m*lda + n] */
+int rsvdt_hip_colsum(int64_t M, int64_t N, const float *A, int64_t lda, float *out, rsvdt_stream_t s);
+/* Alignment loss of ncol learned columns F (ngroups*npts x 2 ncol, re/im
+ * interleaved) with target columns V (ngroups x npts x ncol complex FP64):
+ * loss[g] = scale sum_j (1 - |<v_j, f_j>|^2 / (|f_j|^2 |v_j|^2)); dF its
+ * gradient (d/dRe f + i d/dIm f) in the layout of F.  ncol <= 16. */
+int rsvdt_hip_sketch_loss(int64_t ngroups, int64_t npts, int64_t ncol, const float *F, const double *V, float *dF,
+                          float *loss, double scale, rsvdt_stream_t s);
+/* One Adam step on n parameters; b1t = 1 - beta1^t, b2t = 1 - beta2^t. */
+int rsvdt_hip_adam(int64_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                   float b1t, float b2t, rsvdt_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

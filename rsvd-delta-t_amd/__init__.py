@@ -119,7 +119,8 @@ HIP_ENTRY_POINTS = [
     "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
     "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
-    "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_transpose_rows", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2",
+    "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_transpose_rows", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2", "rsvdt_hip_mlp_features", "rsvdt_hip_gemm_bf16",
+    "rsvdt_hip_colsum", "rsvdt_hip_sketch_loss", "rsvdt_hip_adam",
 ]
 
 _lib = None
