@@ -111,6 +111,10 @@ def main():
               "and frequencies and is not built", file=sys.stderr)
         sys.exit(2)
 
+    # the library's progress messages go to stderr: stdout carries only the JSON line
+    sys.stdout.flush()
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
     pkg = load_pkg()
     if args.calib_copy:
         calib_copy(pkg)
@@ -205,7 +209,8 @@ def main():
         out["time_to_modes_s"] = {"fp64": round(res[64][0], 3), "fp32_state": round(res[32][0], 3),
                                   "speedup": round(res[64][0] / res[32][0], 3), "sigma1_max_rel_err_fp32": err}
     s.close()
-    print(json.dumps(out))
+    sys.stdout.flush()
+    os.write(json_fd, (json.dumps(out) + "\n").encode())
 
 
 if __name__ == "__main__":

@@ -178,6 +178,15 @@ PetscErrorCode SaveResults(Resolvent_matrices *, RSVDt_vars *, Directories *);
 /* replaces SourceCode/SaveInputVarsCopy.h:5 */
 PetscErrorCode SaveInputVarsCopy(Directories *);
 
+/* replaces SourceCode/TransientRunRK4.h:5 (TS.DirAdj selects A or A^H, as in
+ * TSTransRK4; the reference leaves it unset in this mode, the rsvdt driver
+ * sets it to 1) */
+PetscErrorCode TransientRunRK4(TransRun_vars *, RSVDt_vars *, LNS_vars *, DFT_matrices *, Directories *);
+/* replaces SourceCode/TransientRemovalEst.h:5 */
+PetscErrorCode TransientRemovalEst(Mat, Vec, PetscInt, RSVDt_vars *, DFT_matrices *, Directories *);
+/* replaces SourceCode/TSTransRK4.h:5 (TS_matrices unused, may be NULL) */
+PetscErrorCode TSTransRK4(LNS_vars *, TS_matrices *, RSVDt_vars *, PetscInt, Vec);
+
 /* ---- runtime (in place of SlepcInitialize / SlepcFinalize, SourceCode/main.c:107,148) -- */
 /* Parses "-inputs <yaml>" and "-Key value" pairs into the options table,
  * selects the GPU (env RSVDT_DEVICE, default 0) and creates the stream. */
@@ -209,6 +218,14 @@ PetscErrorCode RSVDT_MatDestroy(Mat *m);
 /* MatLoad / MatView (PETSc binary, big-endian, 64-bit or 32-bit indices). */
 PetscErrorCode RSVDT_MatLoad(const char *path, int as_sparse, Mat *out);
 PetscErrorCode RSVDT_MatView(Mat m, const char *path);
+
+/* Vec handles: device-resident complex FP64 vectors (VecCreate/VecLoad/VecView). */
+PetscErrorCode RSVDT_VecCreate(PetscInt n, const double *host, Vec *out);   /* host NULL: zeros */
+PetscErrorCode RSVDT_VecGetValues(Vec v, double *host);
+PetscErrorCode RSVDT_VecGetSize(Vec v, PetscInt *n);
+PetscErrorCode RSVDT_VecDestroy(Vec *v);
+PetscErrorCode RSVDT_VecLoad(const char *path, Vec *out);
+PetscErrorCode RSVDT_VecView(Vec v, const char *path);
 
 /* ---- engine controls (no counterpart in the reference) ------------------- */
 typedef struct {

@@ -81,7 +81,8 @@ typedef struct {
     int a_real;
     int prec;
     int lanes;            /* 0 = default choice */
-    int xcd_remap;        /* 1 = give each XCD a contiguous row range */
+    int xcd_remap;        /* workgroup -> rows map: 0 identity, 1 one contiguous row range per XCD,
+                             2 / 3 chunks of 64 / 256 row blocks dealt round-robin to the XCDs */
     int variant;          /* 0: 16-nonzero gather batches; 1: + nontemporal row-local
                              streams; 2: 8-nonzero batches; 3: 8 + nontemporal; 4: 12-nonzero
                              batches; 5: 12 + nontemporal (variants apply when k <= lanes;
@@ -105,6 +106,13 @@ int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,
  * the launch shape as for rsvdt_hip_rk4_stage. */
 int rsvdt_hip_horner_stage(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
                            const void *g, double c, void *out, double *snap, rsvdt_stream_t s);
+/* As rsvdt_hip_horner_stage, with the snapshot destination read at run time:
+ * when snap_slot is not NULL, the kernel reads the int *snap_slot and writes
+ * the copy of out to snap + slot*snap_stride (complex entries) when slot >= 0,
+ * nowhere when it is negative (graph-captured steps). */
+int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
+                              const void *g, double c, void *out, double *snap, const int *snap_slot,
+                              int64_t snap_stride, rsvdt_stream_t s);
 
 /* Plain block SpMM: out_b = M * in_b for nb blocks of k columns (complex FP64
  * values, int32 CSR).  in blocks are ncols x k (stride in_bstride), out blocks
@@ -207,6 +215,22 @@ int rsvdt_hip_fill_random(int64_t rows, int64_t kb, int64_t nblk, uint64_t seed,
  * dir 1: per-batch -> block.  n*(kb+1) <= 8192.                           */
 int rsvdt_hip_transpose_rows(int64_t nrows, int64_t n, int64_t kt, int64_t b0, int64_t kb, const double *src,
                              double *dst, int dir, rsvdt_stream_t s);
+/* Synthetic snapshot sets of the transient-removal estimate
+ * (SourceCode/TransientRemovalEst.c:84-98): nb per-batch row-major
+ * nrows x ns matrices out[b][r][c] = Q[r*ldq + c + skip] + qss[r]*ph[b*ns + c]
+ * (Q: nrows x ldq row-major snapshots, ph: nb x ns phases), ns + skip <= ldq. */
+int rsvdt_hip_trans_synth(int64_t nrows, int64_t ldq, int64_t ns, int64_t skip, int64_t nb, const double *Q,
+                          const double *qss, const double *ph, double *out, rsvdt_stream_t s);
+/* Snapshot slot of the steps i0 .. i0+S-1 (SourceCode/SaveSnapshots.c:17-29):
+ * slots[t] = the Y_all column of step i0+t, or -1 when that step stores none. */
+int rsvdt_hip_snap_slots(int *slots, int64_t S, int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir,
+                         rsvdt_stream_t s);
+/* HIP graph capture of a stream's launches: begin, end (instantiates into
+ * *exec), launch, destroy. */
+int rsvdt_hip_graph_begin(rsvdt_stream_t s);
+int rsvdt_hip_graph_end(rsvdt_stream_t s, void **exec);
+int rsvdt_hip_graph_launch(void *exec, rsvdt_stream_t s);
+int rsvdt_hip_graph_destroy(void *exec);
 /* sum of |x|^2 over n complex entries into *out (device double) */
 int rsvdt_hip_znorm2(int64_t n, const double *x, double *out_dev, void *work, rsvdt_stream_t s);
 

@@ -74,6 +74,11 @@ def lib():
         L.orc_get_dft.argtypes = [P(_Ctx), _vp, _vp]
         L.orc_time_steps.argtypes = [P(_Ctx), _int, _vp, _i64, _i64]
         L.orc_time_steps.restype = _dbl
+        L.orc_trans_est.argtypes = [P(_Ctx), _vp, _vp, _i64, _vp, _vp]
+        L.orc_trans_est.restype = _int
+        L.orc_trans_run.argtypes = [P(_Ctx), _int, _i64, _i64, _dbl, _dbl, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    P(_dbl)]
+        L.orc_trans_run.restype = _int
         _lib = L
     return _lib
 
@@ -181,6 +186,44 @@ class Oracle:
         Fk = np.asfortranarray(Fk, dtype=np.complex128)
         assert Fk.shape == (self.N, self.Nw_eff)
         return lib().orc_time_steps(ctypes.byref(self.c), int(direct), _ptr(Fk), nvec, nsteps)
+
+    def trans_est(self, Qt, qss, period_index):
+        """TransientRemovalEst.c:26-118 -> (initial norms, updated norms), Nw/2 each."""
+        Qt = np.asfortranarray(Qt, dtype=np.complex128)
+        assert Qt.shape == (self.N, self.Nw)
+        qss = np.ascontiguousarray(qss, dtype=np.complex128)
+        a, b = np.zeros(self.Nw // 2), np.zeros(self.Nw // 2)
+        if lib().orc_trans_est(ctypes.byref(self.c), _ptr(Qt), _ptr(qss), period_index, _ptr(a), _ptr(b)):
+            raise RuntimeError("orc_trans_est failed")
+        return a, b
+
+    def trans_run(self, q0, periods=1, save_mod=100, div_val=1e3, con_val=1e-8, est=False, save=False,
+                  direct=True):
+        """TransientRunRK4.c:39-190.  Returns a dict: norms (Nt_saved), last (N),
+        snaps (Nt_saved x N, when save), init/upd (periods x Nw/2, when est),
+        stop, n_est, Nt_saved, Nt_period, reason (0 none, 1 NaN, 2 diverged,
+        3 converged), dt."""
+        q0 = np.ascontiguousarray(q0, dtype=np.complex128)
+        T = 2 * np.pi / self.c.w
+        dT = T / self.Nw
+        dt2 = dT / np.ceil(dT / self.dt)
+        ntp = int(round(T / dt2))
+        nsv = (ntp * periods) // save_mod + 1
+        norms = np.zeros(nsv)
+        last = np.zeros(self.N, np.complex128)
+        snaps = np.zeros((nsv, self.N), np.complex128) if save else None
+        ini = np.zeros((periods, self.Nw // 2)) if est else None
+        upd = np.zeros((periods, self.Nw // 2)) if est else None
+        info = np.zeros(5, np.int64)
+        d = _dbl()
+        rc = lib().orc_trans_run(ctypes.byref(self.c), int(direct), periods, save_mod, div_val, con_val, int(est),
+                                 _ptr(q0), _ptr(norms), _ptr(last), _ptr(snaps) if save else None,
+                                 _ptr(ini) if est else None, _ptr(upd) if est else None, _ptr(info), ctypes.byref(d))
+        if rc:
+            raise RuntimeError("orc_trans_run failed")
+        assert info[2] == nsv and info[3] == ntp
+        return dict(norms=norms, last=last, snaps=snaps, init=ini, upd=upd, stop=int(info[0]), n_est=int(info[1]),
+                    Nt_saved=nsv, Nt_period=ntp, reason=int(info[4]), dt=d.value)
 
     def spmv_add(self, x, f=None, direct=True):
         x = np.ascontiguousarray(x, dtype=np.complex128)

@@ -671,3 +671,124 @@ double orc_time_steps(orc_ctx *c, int dir, const double *Fk_ri, int64_t nvec, in
     free(y);
     return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
 }
+
+/* ------------------------------------------------------------------ transient run */
+/* TransientRemovalEst.c:26-118.  Qt: N x Nw snapshots of the last period
+ * (column-major), qss: the synthetic steady state.  init_norm[iw] = ||column
+ * iw of Qt*dft/Nw||; upd_norm[iw] = ||qss - P qss|| / ||qss|| with P the
+ * orthogonal projector onto the span of the synthetic snapshots
+ * Qt[:, is + (period_index == 1)] + qss e^{i w iw is deltaT}, is < Nw - (period_index == 1). */
+int orc_trans_est(const orc_ctx *c, const orc_cplx *Qt, const orc_cplx *qss, int64_t period_index,
+                  double *init_norm, double *upd_norm)
+{
+    const int64_t N = c->N, Nw = c->Nw, nh = c->Nw / 2;
+    const double deltaT = 2 * PI_D / c->w / (double)Nw;
+    const int64_t skip = period_index == 1 ? 1 : 0, ns = Nw - skip;
+    for (int64_t iw = 0; iw < nh && init_norm; iw++) {   /* :62-74, Q_hat = Qt dft / Nw */
+        double s2 = 0;
+        for (int64_t r = 0; r < N; r++) {
+            orc_cplx acc = 0;
+            for (int64_t s = 0; s < Nw; s++) acc += Qt[r + N * s] * c->dft[s + Nw * iw];
+            acc *= 1.0 / (double)Nw;
+            s2 += creal(acc) * creal(acc) + cimag(acc) * cimag(acc);
+        }
+        init_norm[iw] = sqrt(s2);
+    }
+    double nq = 0;
+    for (int64_t r = 0; r < N; r++) nq += creal(qss[r]) * creal(qss[r]) + cimag(qss[r]) * cimag(qss[r]);
+    nq = sqrt(nq);
+    orc_cplx *Q = xmalloc(sizeof(orc_cplx) * (size_t)(N * (ns > 0 ? ns : 1)));
+    orc_cplx *b = xmalloc(sizeof(orc_cplx) * (size_t)(ns > 0 ? ns : 1));
+    int rc = 0;
+    for (int64_t iw = 0; iw < nh && upd_norm && !rc; iw++) {   /* :80-114 */
+        const double w = c->w * (double)iw;
+        for (int64_t is = 0; is < ns; is++) {
+            orc_cplx ph = cexp(I * (w * (double)is * deltaT));
+            for (int64_t r = 0; r < N; r++) Q[r + N * is] = Qt[r + N * (is + skip)] + ph * qss[r];
+        }
+        if (orc_qr(N, ns, Q, NULL)) { rc = 1; break; }
+        for (int64_t j = 0; j < ns; j++) {
+            orc_cplx d = 0;
+            for (int64_t r = 0; r < N; r++) d += conj(Q[r + N * j]) * qss[r];
+            b[j] = d;
+        }
+        double s2 = 0;
+        for (int64_t r = 0; r < N; r++) {
+            orc_cplx p = 0;
+            for (int64_t j = 0; j < ns; j++) p += Q[r + N * j] * b[j];
+            orc_cplx d = qss[r] - p;
+            s2 += creal(d) * creal(d) + cimag(d) * cimag(d);
+        }
+        upd_norm[iw] = sqrt(s2) / nq;
+    }
+    free(Q); free(b);
+    return rc;
+}
+
+/* TransientRunRK4.c:39-190: homogeneous RK4 from q0 over TransPeriods periods
+ * (dt re-adjusted to divide T/Nw, :40-44), with the norm monitors every
+ * save_mod steps (NaN / divergence / convergence stop the run before the norm
+ * is recorded) and, when est, the period-end transient-removal estimates on
+ * the last period's Nw snapshots (qss = -q0).  Outputs (any may be NULL):
+ * norms[Nt_saved] (entry 0 = ||q0||, unrecorded entries 0), last[N],
+ * snaps[Nt_saved x N] (the saved snapshots, TransSave), init/upd[periods x Nw/2]
+ * (row p-1 for the estimate at the end of period p); info[0] = the step at
+ * which the run stopped (rend when it ran out), info[1] = estimates made,
+ * info[2] = Nt_saved, info[3] = Nt_period, info[4] = stop reason (0 none,
+ * 1 NaN, 2 diverged, 3 converged); *dt_out = the step used. */
+int orc_trans_run(const orc_ctx *c, int dir, int64_t periods, int64_t save_mod, double div_val, double con_val,
+                  int est, const double *q0_ri, double *norms, double *last_ri, double *snaps_ri,
+                  double *init_norms, double *upd_norms, int64_t *info, double *dt_out)
+{
+    orc_ctx cc = *c;
+    const int64_t N = c->N, Ns = c->Nw;
+    const double TSS = 2 * PI_D / c->w, deltaT = TSS / (double)Ns;
+    cc.dt = deltaT / ceil(deltaT / c->dt);
+    const int64_t Nt_period = (int64_t)round(TSS / cc.dt);
+    const int64_t delta = Nt_period / Ns;
+    const int64_t rend = Nt_period * periods;
+    const int64_t Nt_saved = (int64_t)floor((double)(rend / save_mod)) + 1;
+    if (dt_out) *dt_out = cc.dt;
+    orc_cplx *y = to_cplx(q0_ri, N);
+    orc_cplx *Qt = est ? xcalloc((size_t)(N * Ns), sizeof(orc_cplx)) : NULL;
+    orc_cplx *qss = est ? xmalloc(sizeof(orc_cplx) * (size_t)N) : NULL;
+    if (est) for (int64_t r = 0; r < N; r++) qss[r] = -y[r];
+    double nrm = 0;
+    for (int64_t r = 0; r < N; r++) nrm += creal(y[r]) * creal(y[r]) + cimag(y[r]) * cimag(y[r]);
+    if (norms) { memset(norms, 0, sizeof(double) * (size_t)Nt_saved); norms[0] = sqrt(nrm); }
+    if (snaps_ri) memcpy(snaps_ri, y, sizeof(orc_cplx) * (size_t)N);
+    if (est) memcpy(Qt, y, sizeof(orc_cplx) * (size_t)N);
+    ts_mats t;
+    ts_mats_create(&t, N);
+    int64_t nest = 0, stop = rend, reason = 0;
+    int rc = 0;
+    for (int64_t i = 1; i <= rend && !rc; i++) {
+        ts_trans_rk4(&cc, &t, dir, y);
+        if (i % save_mod == 0) {
+            int64_t pos = i / save_mod;
+            double s2 = 0;
+            for (int64_t r = 0; r < N; r++) s2 += creal(y[r]) * creal(y[r]) + cimag(y[r]) * cimag(y[r]);
+            double nv = sqrt(s2);
+            if (isnan(nv)) { stop = i; reason = 1; break; }
+            if (nv > div_val) { stop = i; reason = 2; break; }
+            if (nv < con_val) { stop = i; reason = 3; break; }
+            if (snaps_ri) memcpy(snaps_ri + 2 * (size_t)pos * (size_t)N, y, sizeof(orc_cplx) * (size_t)N);
+            if (norms) norms[pos] = nv;
+        }
+        if (est && i % delta == 0) {
+            int64_t pos = (i / delta) % Ns;
+            if (pos == 0) {
+                int64_t p = i / Nt_period;
+                rc = orc_trans_est(&cc, Qt, qss, p, init_norms ? init_norms + (p - 1) * (Ns / 2) : NULL,
+                                   upd_norms ? upd_norms + (p - 1) * (Ns / 2) : NULL);
+                nest++;
+            }
+            memcpy(Qt + (size_t)pos * (size_t)N, y, sizeof(orc_cplx) * (size_t)N);
+        }
+    }
+    if (last_ri) memcpy(last_ri, y, sizeof(orc_cplx) * (size_t)N);
+    if (info) { info[0] = stop; info[1] = nest; info[2] = Nt_saved; info[3] = Nt_period; info[4] = reason; }
+    ts_mats_destroy(&t);
+    free(y); free(Qt); free(qss);
+    return rc;
+}

@@ -24,6 +24,10 @@
  *   orc_svd_left            numpy/LAPACK SVD, phase-invariant (test_oracle_linalg)
  *   orc_lu_solve            numpy.linalg.solve (test_oracle_linalg)
  *   orc_rsvdt_run           dense SVD of the RK4 transfer function (test_oracle_e2e)
+ *   orc_trans_run           dense matrix powers of the RK4 step polynomial,
+ *                           stop rules against the closed-form norms (test_oracle_transrun)
+ *   orc_trans_est           FFT of the snapshots and least-squares residuals
+ *                           (numpy lstsq) of the synthetic snapshot sets (test_oracle_transrun)
  *   PGMRES iterate of GalerkinProj.c and SLEPc rounding: parity unpinned
  *   (the reference's PETSc/SLEPc stack cannot be built here; the oracle uses
  *   a direct LU solve and Householder QR whose results are basis-independent).
@@ -116,6 +120,15 @@ double orc_time_steps(orc_ctx *c, int dir, const double *Fk_ri, int64_t nvec, in
 int orc_qr(int64_t m, int64_t n, orc_cplx *a /* m x n col-major, overwritten by Q */, orc_cplx *r /* n x n or NULL */);
 int orc_svd_left(int64_t m, int64_t n, orc_cplx *a /* overwritten by U */, double *s);
 int orc_lu_solve(int64_t n, orc_cplx *a /* n x n col-major, destroyed */, orc_cplx *b /* n, overwritten */);
+
+/* TransientRemovalEst.c:26-118 (Qt: N x Nw column-major snapshots of one
+ * period; init/upd: Nw/2 norms each). */
+int orc_trans_est(const orc_ctx *c, const orc_cplx *Qt, const orc_cplx *qss, int64_t period_index,
+                  double *init_norm, double *upd_norm);
+/* TransientRunRK4.c:39-190 (see rsvdt_oracle.c for the outputs). */
+int orc_trans_run(const orc_ctx *c, int dir, int64_t periods, int64_t save_mod, double div_val, double con_val,
+                  int est, const double *q0_ri, double *norms, double *last_ri, double *snaps_ri,
+                  double *init_norms, double *upd_norms, int64_t *info, double *dt_out);
 
 /* Raw operator applications, for unit tests: y = A x + f (dir=1) or A^H x + f. */
 void orc_spmv_add(const orc_ctx *c, int dir, const orc_cplx *x, const orc_cplx *f, orc_cplx *y);

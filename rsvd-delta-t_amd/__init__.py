@@ -110,17 +110,20 @@ ENTRY_POINTS = [
     "RSVDT_MatCreateDense", "RSVDT_MatCreateDenseBlocked", "RSVDT_MatGetDense", "RSVDT_MatCreateAIJ",
     "RSVDT_MatGetSize", "RSVDT_MatGetAIJ", "RSVDT_MatDestroy", "RSVDT_MatLoad", "RSVDT_MatView", "RSVDT_GetOptions",
     "RSVDT_SetOptions", "RSVDT_GetActionStats", "RSVDT_BenchSteps", "RSVDT_SetSketchColumns",
+    "TransientRunRK4", "TransientRemovalEst", "TSTransRK4", "RSVDT_VecCreate", "RSVDT_VecGetValues",
+    "RSVDT_VecGetSize", "RSVDT_VecDestroy", "RSVDT_VecLoad", "RSVDT_VecView",
 ]
 HIP_ENTRY_POINTS = [
     "rsvdt_hip_init", "rsvdt_hip_last_error", "rsvdt_hip_malloc", "rsvdt_hip_free", "rsvdt_hip_memcpy_h2d",
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
-    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
+    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
+    "rsvdt_hip_graph_begin", "rsvdt_hip_graph_end", "rsvdt_hip_graph_launch", "rsvdt_hip_graph_destroy", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
     "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
     "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_transpose_rows", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2", "rsvdt_hip_mlp_features", "rsvdt_hip_gemm_bf16",
-    "rsvdt_hip_colsum", "rsvdt_hip_sketch_loss", "rsvdt_hip_adam",
+    "rsvdt_hip_colsum", "rsvdt_hip_sketch_loss", "rsvdt_hip_adam", "rsvdt_hip_trans_synth",
 ]
 
 _lib = None
@@ -184,6 +187,15 @@ def lib():
     L.PreProcessing.argtypes = [P(RSVDt_vars), P(Weight_matrices), P(LNS_vars), P(RSVD_matrices),
                                 P(Resolvent_matrices), P(TransRun_vars), P(DFT_matrices), P(Directories)]
     L.SaveResults.argtypes = [P(Resolvent_matrices), P(RSVDt_vars), P(Directories)]
+    L.TransientRunRK4.argtypes = [P(TransRun_vars), P(RSVDt_vars), P(LNS_vars), P(DFT_matrices), P(Directories)]
+    L.TransientRemovalEst.argtypes = [_vp, _vp, _i64, P(RSVDt_vars), P(DFT_matrices), P(Directories)]
+    L.TSTransRK4.argtypes = [P(LNS_vars), _vp, P(RSVDt_vars), _i64, _vp]
+    L.RSVDT_VecCreate.argtypes = [_i64, _vp, P(_vp)]
+    L.RSVDT_VecGetValues.argtypes = [_vp, _vp]
+    L.RSVDT_VecGetSize.argtypes = [_vp, P(_i64)]
+    L.RSVDT_VecDestroy.argtypes = [P(_vp)]
+    L.RSVDT_VecLoad.argtypes = [ctypes.c_char_p, P(_vp)]
+    L.RSVDT_VecView.argtypes = [_vp, ctypes.c_char_p]
     _lib = L
     return L
 
@@ -252,6 +264,45 @@ def destroy(m):
     if m:
         h = _vp(m if isinstance(m, int) else m.value)
         lib().RSVDT_MatDestroy(ctypes.byref(h))
+
+
+def vec_from_numpy(v):
+    """A device Vec holding the complex vector v."""
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    h = _vp()
+    check(lib().RSVDT_VecCreate(v.shape[0], v.ctypes.data_as(_vp), ctypes.byref(h)), "RSVDT_VecCreate")
+    return h
+
+
+def vec_to_numpy(h):
+    n = _i64()
+    check(lib().RSVDT_VecGetSize(h, ctypes.byref(n)), "RSVDT_VecGetSize")
+    v = np.zeros(n.value, np.complex128)
+    check(lib().RSVDT_VecGetValues(h, v.ctypes.data_as(_vp)), "RSVDT_VecGetValues")
+    return v
+
+
+def vec_destroy(h):
+    lib().RSVDT_VecDestroy(ctypes.byref(h))
+
+
+def vec_load(path):
+    """VecLoad of a PETSc binary Vec file -> numpy complex vector."""
+    h = _vp()
+    check(lib().RSVDT_VecLoad(str(path).encode(), ctypes.byref(h)), f"RSVDT_VecLoad({path})")
+    try:
+        return vec_to_numpy(h)
+    finally:
+        vec_destroy(h)
+
+
+def vec_view(v, path):
+    """VecView of a numpy vector into a PETSc binary file."""
+    h = vec_from_numpy(v)
+    try:
+        check(lib().RSVDT_VecView(h, str(path).encode()), f"RSVDT_VecView({path})")
+    finally:
+        vec_destroy(h)
 
 
 def get_options():
@@ -393,6 +444,52 @@ class Session:
                                      ctypes.byref(self.D), nsteps, warmup, ctypes.byref(ms), ctypes.byref(rk),
                                      ctypes.byref(fr)), "RSVDT_BenchSteps")
         return ms.value, rk.value, fr.value
+
+    def transient_run(self, folder, periods=1, save_mod=100, div_val=1e3, con_val=1e-8, est=False, save=False,
+                      q0=None, seed=1373, direct=True):
+        """TransientRunRK4 (SourceCode/TransientRunRK4.c:39-227) with this
+        session's operator, w, Nw and dt: files go to `folder`; q0 (optional)
+        is written there as the initial vector (TransICFlg), else a seeded
+        random vector is used.  Returns the dt the run used."""
+        L = lib()
+        folder = os.path.join(os.path.abspath(folder), "")
+        os.makedirs(folder, exist_ok=True)
+        TR = TransRun_vars()
+        TR.TransRun, TR.TransSave, TR.TransRemovalEst = 1, int(bool(save)), int(bool(est))
+        TR.TransSaveMod, TR.TransPeriods, TR.TransDivVal, TR.TransConVal = save_mod, periods, div_val, con_val
+        V = RSVDt_vars.from_buffer_copy(self.V)
+        V.RSVD.RandSeed = seed
+        V.TS.DirAdj = int(bool(direct))
+        dirs = Directories()
+        dirs.FolderDir = folder.encode()
+        if q0 is not None:
+            TR.TransICFlg = 1
+            vec_view(q0, os.path.join(folder, "q0_input"))
+            dirs.RootDir = folder.encode()
+            dirs.TransICDir = b"q0_input"
+        check(L.TransientRunRK4(ctypes.byref(TR), ctypes.byref(V), ctypes.byref(self.LNS), ctypes.byref(self.D),
+                                ctypes.byref(dirs)), "TransientRunRK4")
+        return V.TS.dt
+
+    def trans_removal_est(self, Qt, qss, period_index, folder):
+        """TransientRemovalEst (SourceCode/TransientRemovalEst.c:26-127) on
+        N x Nw snapshots Qt; returns (initial norms, updated norms) read back
+        from the files it writes into `folder`."""
+        folder = os.path.join(os.path.abspath(folder), "")
+        os.makedirs(folder, exist_ok=True)
+        dirs = Directories()
+        dirs.FolderDir = folder.encode()
+        m = mat_from_dense(np.asarray(Qt, dtype=np.complex128), kb=1)
+        v = vec_from_numpy(qss)
+        try:
+            check(lib().TransientRemovalEst(m, v, period_index, ctypes.byref(self.V), ctypes.byref(self.D),
+                                            ctypes.byref(dirs)), "TransientRemovalEst")
+        finally:
+            destroy(m)
+            vec_destroy(v)
+        a = vec_load(os.path.join(folder, f"Initial_transient_norm_period_{period_index}"))
+        b = vec_load(os.path.join(folder, f"Updated_transient_norm_period_{period_index}"))
+        return a.real, b.real
 
     def close(self):
         for h in (self.R.Y_hat, self.R.F_hat, self.Res.U_hat, self.Res.V_hat, self.Res.S_hat, self.LNS.A,

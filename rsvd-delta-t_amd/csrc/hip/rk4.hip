@@ -17,10 +17,36 @@
 // complex values; LPR lanes (power of two >= k, capped at 64) cooperate on
 // one row and each lane owns CPL columns.  With xcd_remap the workgroups of
 // one XCD (dispatch is round-robin over the 8 XCDs) get a contiguous row
-// range, so the stencil neighbours a row gathers sit in that XCD's L2.
+// range (or interleaved chunks of row blocks, see xcd_block), so the stencil
+// neighbours a row gathers sit in that XCD's L2.
 #include "common.h"
 
 namespace rsvdt {
+
+// Workgroup -> row-block map.  Dispatch hands workgroup i to XCD i % 8.
+// mode 0: identity.  mode 1: XCD x gets one contiguous range of row blocks
+// (its L2 sees the stencil neighbours, but the 8 XCDs sweep 8 distant parts
+// of the vector at once).  modes 2, 3: XCD x gets chunks x, x+8, x+16, ... of
+// 64 (mode 2) or 256 (mode 3) row blocks, so all XCDs sweep the same region
+// of the vector together (one reuse window in the shared last-level cache)
+// while each chunk's near neighbours stay in one L2; the tail that does not
+// fill a whole round of 8 chunks keeps the identity map.
+__device__ __forceinline__ int xcd_block(int bx, int nb, int mode)
+{
+    if (mode == 1) {
+        const int q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
+        return x * q + (x < r ? x : r) + idx;
+    }
+    if (mode >= 2) {
+        const int ch = mode == 2 ? 64 : 256;
+        const int full = nb / (8 * ch);
+        if (bx < full * 8 * ch) {
+            const int x = bx & 7, idx = bx >> 3;
+            return ((idx / ch) * 8 + x) * ch + idx % ch;
+        }
+    }
+    return bx;
+}
 
 // Matrix values are FP64 for both state precisions (real or complex); the
 // SpMM accumulates in FP64 and only the stored state vectors are FP32 in the
@@ -136,11 +162,7 @@ __global__ __launch_bounds__(256) void k_rk4_stage(
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
     constexpr int RPB = 256 / LPR;                  // rows per workgroup
-    int bx = blockIdx.x;
-    if (xcd_remap) {
-        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
-        bx = x * q + (x < r ? x : r) + idx;
-    }
+    const int bx = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
     const int lane = threadIdx.x & (LPR - 1);
     const int row = bx * RPB + (int)(threadIdx.x / LPR);
     if (row >= nrows) return;
@@ -204,7 +226,7 @@ static int launch_rk4(const rsvdt_spmm_desc *d, int stage, const void *uin, cons
     constexpr int RPB = 256 / LPR;
     const unsigned gx = ceil_div(d->nrows, RPB);
     const unsigned gy = (unsigned)d->nb;
-    const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
+    const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
     hipLaunchKernelGGL((k_rk4_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s,
                        (int)d->nrows, (int)d->k, stage, remap, d->nrows * d->k, d->rowptr, d->col, d->val,
                        (const V *)uin, (const V *)y, (const V *)F, (V *)acc, (V *)out, cn, ca,
@@ -257,22 +279,23 @@ __global__ __launch_bounds__(256) void k_horner_stage(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
     const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
     const double2 *__restrict__ ys, const typename vec2<T>::type *__restrict__ g,
-    typename vec2<T>::type *out, double c, double2 *__restrict__ snap)
+    typename vec2<T>::type *out, double c, double2 *__restrict__ snap, const int *__restrict__ snap_slot,
+    int64_t snap_stride)
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
     constexpr int RPB = 256 / LPR;
-    int bx = blockIdx.x;
-    if (xcd_remap) {
-        const int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
-        bx = x * q + (x < r ? x : r) + idx;
-    }
+    const int bx = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
     const int lane = threadIdx.x & (LPR - 1);
     const int row = bx * RPB + (int)(threadIdx.x / LPR);
     if (row >= nrows) return;
     const int64_t boff = (int64_t)blockIdx.y * bstride;
     const AV *vals = (const AV *)vals_;
     const int e0 = rowptr[row], e1 = rowptr[row + 1];
+    if (snap_slot != nullptr) {                 // snapshot slot read at run time (graph replays)
+        const int sl = *snap_slot;
+        snap = sl >= 0 ? snap + (int64_t)sl * snap_stride : nullptr;
+    }
 
     V yr[CPL], gr[CPL];
     // when y is also the gathered vector (first stage) its lines are reused
@@ -317,23 +340,23 @@ __global__ __launch_bounds__(256) void k_horner_stage(
 
 template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
 static int launch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
-                         double c, void *out, double *snap, hipStream_t s)
+                         double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
     using V = typename vec2<T>::type;
     constexpr int RPB = 256 / LPR;
     const unsigned gx = ceil_div(d->nrows, RPB);
     const unsigned gy = (unsigned)d->nb;
-    const int remap = (d->xcd_remap && gy == 1 && gx >= 8) ? 1 : 0;
+    const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
     hipLaunchKernelGGL((k_horner_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows,
                        (int)d->k, remap, d->nrows * d->k, d->rowptr, d->col, d->val, (const V *)vin, (const V *)y,
-                       (const double2 *)ys, (const V *)g, (V *)out, c, (double2 *)snap);
+                       (const double2 *)ys, (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
     return RSVDT_HIP_OK;
 }
 
 template <typename T, bool AREAL>
 static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
-                           double c, void *out, double *snap, hipStream_t s)
+                           double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
     int lpr = d->lanes;
     const int k = (int)d->k;
@@ -346,13 +369,13 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
 #define RSVDT_HCASE(L, C)                                                                                         \
     if (lpr == L && cpl <= C) {                                                                                   \
         if constexpr (C == 1 && L >= 16) {                                                                        \
-            if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, s);     \
-            if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, s);      \
+            if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);     \
+            if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);      \
         }                                                                                                         \
-        if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, s); \
-        if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, s); \
-        if (C == 1 && var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, s);  \
-        return launch_horner<T, AREAL, L, C, 16, false>(d, vin, y, ys, g, c, out, snap, s);                        \
+        if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
+        if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
+        if (C == 1 && var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);  \
+        return launch_horner<T, AREAL, L, C, 16, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);                        \
     }
     RSVDT_HCASE(4, 1) RSVDT_HCASE(8, 1) RSVDT_HCASE(16, 1) RSVDT_HCASE(32, 1) RSVDT_HCASE(64, 1)
     RSVDT_HCASE(4, 2) RSVDT_HCASE(8, 2) RSVDT_HCASE(16, 2) RSVDT_HCASE(32, 2) RSVDT_HCASE(64, 2)
@@ -407,26 +430,33 @@ static int check_desc(const rsvdt_spmm_desc *d, const char *who)
     return RSVDT_HIP_OK;
 }
 
-extern "C" int rsvdt_hip_horner_stage(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
-                                      const void *g, double c, void *out, double *snap, rsvdt_stream_t s)
+extern "C" int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *vin, const void *y,
+                                         const double *yscale, const void *g, double c, void *out, double *snap,
+                                         const int *snap_slot, int64_t snap_stride, rsvdt_stream_t s)
 {
     int rc = check_desc(d, "rsvdt_hip_horner_stage");
     if (rc) return rc;
-    if (!vin || !y || !out) {
-        set_error_msg("rsvdt_hip_horner_stage: vin, y and out are required");
+    if (!vin || !y || !out || (snap_slot && !snap)) {
+        set_error_msg("rsvdt_hip_horner_stage: vin, y and out (and snap with a slot) are required");
         return RSVDT_HIP_ERR_ARG;
     }
     hipStream_t st = as_stream(s);
     if (d->prec == 64) {
-        if (d->a_real) return dispatch_horner<double, true>(d, vin, y, yscale, g, c, out, snap, st);
-        return dispatch_horner<double, false>(d, vin, y, yscale, g, c, out, snap, st);
+        if (d->a_real) return dispatch_horner<double, true>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
+        return dispatch_horner<double, false>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
     }
     if (d->prec == 32) {
-        if (d->a_real) return dispatch_horner<float, true>(d, vin, y, yscale, g, c, out, snap, st);
-        return dispatch_horner<float, false>(d, vin, y, yscale, g, c, out, snap, st);
+        if (d->a_real) return dispatch_horner<float, true>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
+        return dispatch_horner<float, false>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
     }
     set_error_msg("rsvdt_hip_horner_stage: prec must be 64 or 32");
     return RSVDT_HIP_ERR_ARG;
+}
+
+extern "C" int rsvdt_hip_horner_stage(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *yscale,
+                                      const void *g, double c, void *out, double *snap, rsvdt_stream_t s)
+{
+    return rsvdt_hip_horner_stage_ex(d, vin, y, yscale, g, c, out, snap, nullptr, 0, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -255,6 +255,97 @@ extern "C" int rsvdt_hip_transpose_rows(int64_t nrows, int64_t n, int64_t kt, in
     return RSVDT_HIP_OK;
 }
 
+// Synthetic snapshot sets of the transient-removal estimate
+// (SourceCode/TransientRemovalEst.c:84-98), per-batch row-major:
+// out[b][r][c] = Q[r*ldq + c + skip] + qss[r] * ph[b*ns + c], c < ns.
+__global__ __launch_bounds__(256) void k_trans_synth(int64_t nrows, int64_t ldq, int64_t ns, int64_t skip,
+                                                     int64_t total, const double2 *__restrict__ Q,
+                                                     const double2 *__restrict__ qss, const double2 *__restrict__ ph,
+                                                     double2 *__restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t c = e % ns, rb = e / ns, r = rb % nrows, b = rb / nrows;
+        const double2 q = Q[r * ldq + c + skip], v = qss[r], p = ph[b * ns + c];
+        out[e] = make_double2(q.x + (v.x * p.x - v.y * p.y), q.y + (v.x * p.y + v.y * p.x));
+    }
+}
+extern "C" int rsvdt_hip_trans_synth(int64_t nrows, int64_t ldq, int64_t ns, int64_t skip, int64_t nb,
+                                     const double *Q, const double *qss, const double *ph, double *out,
+                                     rsvdt_stream_t s)
+{
+    if (nrows <= 0 || ns <= 0 || nb <= 0) return RSVDT_HIP_OK;
+    if (skip < 0 || ns + skip > ldq) {
+        set_error_msg("rsvdt_hip_trans_synth: bad arguments (ns + skip <= ldq)");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    const int64_t total = nb * nrows * ns;
+    const int64_t g = total / 256 + 1 < 65536 ? total / 256 + 1 : 65536;
+    hipLaunchKernelGGL(k_trans_synth, dim3((unsigned)g), dim3(256), 0, as_stream(s), nrows, ldq, ns, skip, total,
+                       (const double2 *)Q, (const double2 *)qss, (const double2 *)ph, (double2 *)out);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_trans_synth");
+    return RSVDT_HIP_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+// Snapshot slot of each step of a chunk (SourceCode/SaveSnapshots.c:17-29):
+// step i = i0 + t stores a snapshot when i > Nt and (i - Nt) % RR == 0, at
+// column (i - Nt)/RR - 1 mod Nstore (reversed for the adjoint); -1 otherwise.
+__global__ void k_snap_slots(int *slots, int64_t S, int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S) return;
+    const int64_t i = i0 + t;
+    int v = -1;
+    if (i > Nt && (i - Nt) % RR == 0) {
+        int64_t pos = ((i - Nt) / RR - 1) % Nstore;
+        if (!dir) pos = Nstore - 1 - pos;
+        v = (int)pos;
+    }
+    slots[t] = v;
+}
+extern "C" int rsvdt_hip_snap_slots(int *slots, int64_t S, int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore,
+                                    int dir, rsvdt_stream_t s)
+{
+    if (S <= 0) return RSVDT_HIP_OK;
+    if (RR <= 0 || Nstore <= 0) { set_error_msg("rsvdt_hip_snap_slots: bad arguments"); return RSVDT_HIP_ERR_ARG; }
+    hipLaunchKernelGGL(k_snap_slots, dim3(ceil_div(S, 256)), dim3(256), 0, as_stream(s), slots, S, i0, Nt, RR, Nstore,
+                       dir);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_snap_slots");
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_graph_begin(rsvdt_stream_t s)
+{
+    RSVDT_HIP_TRY(hipStreamBeginCapture(as_stream(s), hipStreamCaptureModeThreadLocal));
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_graph_end(rsvdt_stream_t s, void **exec)
+{
+    hipGraph_t g = nullptr;
+    *exec = nullptr;
+    RSVDT_HIP_TRY(hipStreamEndCapture(as_stream(s), &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) { set_error("hipGraphInstantiate", err); return RSVDT_HIP_ERR_RUNTIME; }
+    *exec = (void *)e;
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_graph_launch(void *exec, rsvdt_stream_t s)
+{
+    RSVDT_HIP_TRY(hipGraphLaunch((hipGraphExec_t)exec, as_stream(s)));
+    return RSVDT_HIP_OK;
+}
+
+extern "C" int rsvdt_hip_graph_destroy(void *exec)
+{
+    if (exec) RSVDT_HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)exec));
+    return RSVDT_HIP_OK;
+}
+
 // sum |x|^2: per-block partials into work, then one block sums them (deterministic)
 __global__ void k_znorm2_part(int64_t n, const double2 *x, double *part) {
     __shared__ double sh[256];

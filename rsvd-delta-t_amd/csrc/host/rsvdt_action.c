@@ -18,7 +18,7 @@
 
 /* scratch slots */
 enum { SCR_Y = 0, SCR_U1, SCR_U2, SCR_ACC, SCR_FBUF, SCR_FBUF2, SCR_G, SCR_R, SCR_RINV, SCR_RACC, SCR_RTMP,
-       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_FSCALE, SCR_W1, SCR_W2, SCR_QRT, SCR_GINFO, SCR_NSLOTS };
+       SCR_WORK, SCR_INFO, SCR_MISC, SCR_MISC2, SCR_PROJ, SCR_MASK, SCR_RMASK, SCR_CNT, SCR_FSCALE, SCR_W1, SCR_W2, SCR_QRT, SCR_GINFO, SCR_SLOTS, SCR_NSLOTS };
 _Static_assert(SCR_NSLOTS <= RSVDT_NSCRATCH, "scratch slot table too small");
 
 /* ------------------------------------------------------------------ stepper */
@@ -72,7 +72,7 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->a_real = A->is_real;
     d->prec = prec;
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
-    if (A->tuned_ok[f][p][q] && nb == 1) {
+    if (A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k && nb == 1) {
         d->lanes = A->tuned_lanes[f][p][q];
         d->variant = A->tuned_variant[f][p][q];
         d->xcd_remap = A->tuned_remap[f][p][q];
@@ -190,6 +190,35 @@ static PetscErrorCode stepper_step(stepper *st, int buf, PetscInt s, double *sna
     return 0;
 }
 
+/* The S steps of a full chunk from buffer buf (form 0) as one HIP graph: the
+ * snapshot column of each step is read by the last stage from slots[t]
+ * (filled per chunk before the graph runs), so the same graph serves every
+ * chunk of its buffer. */
+static PetscErrorCode stepper_capture(stepper *st, int buf, const int *slots, double *snap_base, void **exec)
+{
+    const double h = st->h;
+    rsvdt_stream_t q = g_rsvdt.stream;
+    const rsvdt_spmm_desc *d = &st->desc;
+    const int64_t sstride = (int64_t)st->N * (int64_t)st->k;
+    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_graph_begin(q));
+    int rc = 0;
+    for (PetscInt t = 0; t < st->S && !rc; t++) {
+        rc = rsvdt_hip_horner_stage(d, st->y, st->y, NULL, NULL, h / 4, st->u1, NULL, q);
+        if (!rc) rc = rsvdt_hip_horner_stage(d, st->u1, st->y, NULL, NULL, h / 3, st->u2, NULL, q);
+        if (!rc) rc = rsvdt_hip_horner_stage(d, st->u2, st->y, NULL, NULL, h / 2, st->u1, NULL, q);
+        if (!rc) rc = rsvdt_hip_horner_stage_ex(d, st->u1, st->y, NULL, fslot(st, buf, t), h, st->y, snap_base,
+                                                snap_base ? slots + t : NULL, sstride, q);
+    }
+    void *e = NULL;
+    int rc2 = rsvdt_hip_graph_end(q, &e);         /* always end the capture */
+    if (rc || rc2) {
+        if (e) rsvdt_hip_graph_destroy(e);
+        return rsvdt_err(rc ? rc : rc2, "TSActionRK4", "graph capture failed: %s", rsvdt_hip_last_error());
+    }
+    *exec = e;
+    return 0;
+}
+
 /* G_m = Q(z_m) F_hat_m for the Nwe frequency blocks (form 0), FP64, written
  * to G (which may be F itself: the last stage reads F row-locally only).
  * z_m = Ns * idft(m, +-1) is the iDFT's own half-step phase of frequency m.
@@ -296,7 +325,7 @@ static void pt_free(phase_timer *t)
 static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
 {
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
-    if (A->tuned_ok[f][p][q] || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
+    if ((A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k) || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
     rsvdt_spmm_desc dsc;
     desc_for(&dsc, A, dir, prec, k, 1);
     void *e0, *e1;
@@ -310,7 +339,7 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         const int cpl = (int)((k + l - 1) / l);
         const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? 6 : 4);
         for (int v = 0; v < nvar; v++)
-            for (int r = 0; r < 2; r++) {
+            for (int r = 0; r < 4; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
                 /* all operands are scratch buffers: gathered a, row-local b (and d), out c */
 #define RSVDT_TUNE_LAUNCH()                                                                              \
@@ -332,6 +361,7 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
     A->tuned_variant[f][p][q] = bv;
     A->tuned_remap[f][p][q] = br;
     A->tuned_ok[f][p][q] = 1;
+    A->tuned_k[f][p][q] = k;
     if (g_rsvdt.opt.verbose)
         rsvdt_printf("[rsvdt] autotune (%s, FP%d, %s): %d lanes/row, variant %d, xcd remap %d: %.1f us per stage\n",
                      dir ? "A" : "A^H", prec, f ? "stage form" : "Horner form", bl, bv, br, 1e3 * best / 3);
@@ -379,12 +409,33 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         if (t_frc) pt_end_on(t_frc, frc_stream());
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
     }
+    /* graph replay of full chunks (form 0): one graph per buffer */
+    const int use_graph = g_rsvdt.opt.use_graph && st.form == 0;
+    void *gexec[2] = {NULL, NULL};
+    int *slots = NULL;
+    if (use_graph) {
+        slots = rsvdt_scratch(SCR_SLOTS, sizeof(int) * 2 * (size_t)st.S);
+        if (!slots) { stepper_free(&st); return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate slot table"); }
+    }
     for (PetscInt c = 0; c < nch && !ierr; c++) {
         const int buf = (int)(c & 1);
         const PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
         if (t_rk4) pt_begin(t_rk4);
-        for (PetscInt s = 0; s < steps; s++) {
+        if (use_graph && steps == st.S) {
+            int *sl = slots + buf * st.S;
+            if (Yall) {
+                int hr = rsvdt_hip_snap_slots(sl, st.S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream);
+                if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
+            }
+            if (!gexec[buf]) ierr = stepper_capture(&st, buf, sl, Yall ? Yall->d : NULL, &gexec[buf]);
+            if (!ierr) {
+                int hr = rsvdt_hip_graph_launch(gexec[buf], g_rsvdt.stream);
+                if (hr) ierr = rsvdt_err(hr, "TSActionRK4", "graph launch failed: %s", rsvdt_hip_last_error());
+            }
+            launches += 4 * steps;
+        }
+        for (PetscInt s = 0; s < steps && !(use_graph && steps == st.S); s++) {
             PetscInt i = i0 + s;
             double *snap = NULL;
             if (Yall && i > V->TS.Nt && (i - V->TS.Nt) % V->TS.ResRatio == 0) {
@@ -410,6 +461,7 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     }
     /* the main stream must not run ahead of a forcing GEMM still reading F_hat */
     RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_sync());
+    for (int b = 0; b < 2; b++) rsvdt_hip_graph_destroy(gexec[b]);
     stepper_free(&st);
     RSVDT_CHK(ierr);
     g_rsvdt.stats.steps = rend;

@@ -38,8 +38,9 @@ struct _p_RSVDT_Mat {
     void *d_val64, *d_valH64;
     /* autotuned SpMM launch shape per [stepping form][precision 64/32]
      * [direct/adjoint]: lanes per row, kernel variant, XCD remap; tuned_ok
-     * marks a tuned entry */
+     * marks a tuned entry, valid for blocks of tuned_k columns */
     int tuned_lanes[2][2][2], tuned_variant[2][2][2], tuned_remap[2][2][2], tuned_ok[2][2][2];
+    PetscInt tuned_k[2][2][2];   /* block width the entry was tuned for */
 };
 
 struct _p_RSVDT_Vec {

@@ -28,10 +28,11 @@ int main(int argc, char **argv)
     memset(&TSR, 0, sizeof(TSR)); memset(&W, 0, sizeof(W));
     CHK(RSVDT_Initialize(&argc, &argv));
     CHK(PreProcessing(&RSVDt, &W, &LNS, &R, &Res, &TR, &DFTm, &dirs));
-    if (TR.TransRun) {
-        fprintf(stderr, "[rsvdt] TransRun (the transient-simulation pre-processing mode) is not part of this engine\n");
-        RSVDT_Finalize();
-        return 56;
+    if (TR.TransRun) {   /* SourceCode/main.c:119-124: run and exit; the forward operator A */
+        RSVDt.TS.DirAdj = PETSC_TRUE;
+        CHK(TransientRunRK4(&TR, &RSVDt, &LNS, &DFTm, &dirs));
+        RSVDT_MatDestroy(&LNS.A); RSVDT_MatDestroy(&DFTm.idft); RSVDT_MatDestroy(&DFTm.dft);
+        return RSVDT_Finalize();
     }
     printf("\n********************************************\n*************** RSVD-\\Delta t **************\n"
            "********************************************\n");

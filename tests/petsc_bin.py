@@ -63,3 +63,16 @@ def write_vec(path, v):
     with open(path, "wb") as f:
         f.write(np.array([VEC_FILE_CLASSID, len(v)], dtype=">i8").tobytes())
         f.write(v.astype(">c16").tobytes())
+
+
+def read_vec(path):
+    """PETSc binary Vec (64-bit header, complex scalars) -> complex vector."""
+    with open(path, "rb") as f:
+        hdr = np.frombuffer(f.read(16), dtype=">i8")
+        if hdr[0] != VEC_FILE_CLASSID:
+            raise ValueError(f"{path}: not a PETSc Vec (classid {hdr[0]})")
+        n = int(hdr[1])
+        v = np.frombuffer(f.read(16 * n), dtype=">c16").astype(np.complex128)
+        if v.shape[0] != n or f.read(1):
+            raise ValueError(f"{path}: size mismatch")
+    return v

@@ -219,6 +219,34 @@ def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced)
     assert np.array_equal(snap, out.astype(np.complex128))
 
 
+@pytest.mark.parametrize("k,lanes", [(16, 16), (32, 32), (4, 4)])
+def test_horner_stage_xcd_remaps(L, pkg, k, lanes):
+    """Every workgroup-to-rows map (identity, one range per XCD, chunks of 64
+    or 256 row blocks dealt to the XCDs, with a tail that fills no whole
+    round) computes every row once: identical results, equal to numpy."""
+    print(f"horner remaps k={k} lanes={lanes}", flush=True)
+    rng = np.random.default_rng(k)
+    N = 40000 + 7
+    A = sp.random(N, N, density=5.0 / N, random_state=k, format="csr", dtype=np.float64)
+    A = (A - sp.eye(N)).tocsr()
+    A.sort_indices()
+    vin, y = crandn(rng, N, k), crandn(rng, N, k)
+    ref = y + 0.05 * (A @ vin)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32))
+    dv = Dev(pkg, A.data.astype(np.float64))
+    dvin, dy = Dev(pkg, vin), Dev(pkg, y)
+    outs = []
+    for remap in (0, 1, 2, 3):
+        dout = Dev(pkg, nbytes=y.nbytes)
+        d = Desc(N, k, 1, drp.p, dci.p, dv.p, 1, 64, lanes, remap, 3)
+        L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
+        assert L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, None, None, 0.05, dout.p, None, None) == 0
+        outs.append(dout.get(np.complex128, y.shape))
+    assert rel(outs[0], ref) < 1e-14
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def test_horner_stage_rejects_bad_desc(L, pkg):
     d = Desc(0, 16, 1, None, None, None, 1, 64, 0, 1, 0)
     L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
