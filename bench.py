@@ -89,6 +89,31 @@ def calib_copy(pkg, gib=1, reps=5):
                                      "launches": reps, "kernel": "k_zcopy3"}}))
 
 
+def time_to_modes_gl(pkg, nmembers):
+    """Time to sigma_1 relative error <= 1e-6 on held-out members of the
+    Ginzburg-Landau family (N=500, k=16, Nw=64, transient removal): the plain
+    FP64 GPU form (reference stage order, one launch per stage, random
+    sketch) against the optimised form (Horner stages, LDS-resident chunks,
+    learned sketch columns from the trained network), each with the fewest
+    power iterations that reach the tolerance."""
+    import learned
+    if not os.path.exists(learned.WEIGHTS):
+        return {"skipped": f"no trained weights at {os.path.relpath(learned.WEIGHTS, ROOT)}"}
+    net = learned.SketchNet(seed=0).load(learned.WEIGHTS)
+    members = learned.family_sample(np.random.default_rng(777), nmembers)
+    rows, summ = learned.time_to_modes(pkg, net, members)
+    summ = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
+    summ["members"] = [{"mu0": round(r["mu0"], 4), "ReU": round(r["Ur"], 4),
+                        "plain": {"q": r["plain"]["q"], "s": round(r["plain"]["wall_s"], 4),
+                                  "err": r["plain"]["sigma1_err"]},
+                        "fast": {"q": r["fast"]["q"], "s": round(r["fast"]["wall_s"], 4),
+                                 "err": r["fast"]["sigma1_err"]}} for r in rows]
+    summ["workload"] = ("Ginzburg-Landau family (N=500, k=16 = 8 learned + 8 random columns, Nw=64, w=0.05, "
+                        "dt=T/4096, TL=10, transient removal), held-out members; tolerance 1e-6 on sigma_1 "
+                        "against q=4 of the optimised form")
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +130,9 @@ def main():
                     help="0: Horner stages with the folded forcing (default); 1: the reference's stage order")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (profiler counter calibration)")
+    ap.add_argument("--no-ttm", action="store_true",
+                    help="skip the time-to-modes comparison on held-out Ginzburg-Landau family members")
+    ap.add_argument("--ttm-members", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards test vectors "
@@ -159,6 +187,14 @@ def main():
     else:
         bytes_step_alg = 4 * S_A + 20 * V + (2 * Nwe / (2 * S + 1) + 2) * V
         kernel, bytes_launch = "k_rk4_stage", S_A + 5 * V
+    # HBM traffic per launch of the stage kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload and precision
+    traffic, traffic_src = None, None
+    tj = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tj) and form == 0:
+        ent = json.load(open(tj)).get(f"{args.workload}_fp{args.precision}")
+        if ent:
+            traffic, traffic_src = ent["traffic_bytes"], "profiles/" + ent["profile"]
     ms_step = ms / args.steps
     value = bytes_step_ref / (ms_step * 1e-3) / 1e9
     # dominant kernel: the fused SpMM stage kernel, 4 launches per step
@@ -187,7 +223,7 @@ def main():
                    "spmm_shape": {"lanes_per_row": st["spmm_lanes"], "variant": st["spmm_variant"],
                                   "xcd_remap": st["spmm_xcd_remap"]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kernel, "bytes_per_launch": int(bytes_launch),
                      "avg_launch_ms": round(launch_ms, 5)},
         "breakdown_ms_per_step": {"rk4": round(rk_ms / args.steps, 5), "forcing": round(fr_ms / args.steps, 5)},
@@ -209,6 +245,8 @@ def main():
         out["time_to_modes_s"] = {"fp64": round(res[64][0], 3), "fp32_state": round(res[32][0], 3),
                                   "speedup": round(res[64][0] / res[32][0], 3), "sigma1_max_rel_err_fp32": err}
     s.close()
+    if not args.no_ttm and args.workload == "flow2d":
+        out["time_to_modes"] = time_to_modes_gl(pkg, args.ttm_members)
     sys.stdout.flush()
     os.write(json_fd, (json.dumps(out) + "\n").encode())
 

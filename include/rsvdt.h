@@ -239,6 +239,7 @@ typedef struct {
     int overlap_forcing;  /* compute the next forcing chunk on a second stream during the RK4 steps */
     int rk4_form;         /* 0: RK4 as y' = P(hA) y + g (Horner stages, forcing folded into one
                              precomputed term per step); 1: the reference's stage order */
+    int fuse_small;       /* operators whose state fits in LDS: whole chunks of steps per launch */
 } RSVDT_Options;
 PetscErrorCode RSVDT_GetOptions(RSVDT_Options *o);
 PetscErrorCode RSVDT_SetOptions(const RSVDT_Options *o);

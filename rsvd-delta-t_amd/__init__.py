@@ -90,7 +90,8 @@ class Directories(ctypes.Structure):
 
 class RSVDT_Options(ctypes.Structure):
     _fields_ = [("precision", _int), ("chunk_steps", _int), ("spmm_lanes", _int), ("xcd_remap", _int),
-                ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int), ("rk4_form", _int)]
+                ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int), ("rk4_form", _int),
+                ("fuse_small", _int)]
 
 
 class RSVDT_ActionStats(ctypes.Structure):
@@ -118,7 +119,7 @@ HIP_ENTRY_POINTS = [
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
-    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
+    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_fused", "rsvdt_hip_horner_fused_max_rows", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
     "rsvdt_hip_graph_begin", "rsvdt_hip_graph_end", "rsvdt_hip_graph_launch", "rsvdt_hip_graph_destroy", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
     "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
@@ -313,7 +314,7 @@ def get_options():
 
 # engine option defaults (RSVDT_Initialize's); a Session starts from these
 OPTION_DEFAULTS = dict(precision=64, chunk_steps=0, spmm_lanes=0, xcd_remap=1, use_graph=0, autotune=1, verbose=0,
-                       overlap_forcing=1, rk4_form=0)
+                       overlap_forcing=1, rk4_form=0, fuse_small=1)
 
 
 def set_options(**kw):

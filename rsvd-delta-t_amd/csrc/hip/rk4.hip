@@ -21,6 +21,9 @@
 // neighbours a row gathers sit in that XCD's L2.
 #include "common.h"
 
+// LDS budget of the fused small-operator stepper (y and two stage vectors)
+#define RSVDT_FUSED_LDS (96 * 1024)
+
 namespace rsvdt {
 
 // Workgroup -> row-block map.  Dispatch hands workgroup i to XCD i % 8.
@@ -86,66 +89,83 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 v, bool nt) {
     else *p = v;
 }
 
-// s[t] += sum over the row's nonzeros e of A(row, col_e) * x(col_e, lane + t*LPR):
-// the row's (column, value) pairs are fetched NB at a time with one coalesced
-// load per lane and broadcast by shuffles, then the NB gathered rows of x are
-// loaded back to back before any FMA.
+// s[t] += sum over the row's nonzeros e of A(row, col_e) * x(col_e, lane + t*LPR).
+// One batch of a row's nonzeros eb .. eb+min(NB, e1-eb)-1: column indices
+// and values are fetched cooperatively (one coalesced load per lane) and
+// broadcast with __shfl; then all NB gathers are issued before any FMA, so
+// NB independent requests are in flight per lane.  Gathers past the row's
+// end read row 0 (valid) and are not used; lanes beyond k read column k-1
+// and their sums are never stored.
+template <typename T, bool AREAL, int LPR, int CPL, int NB>
+__device__ __forceinline__ void gather_batch(int eb, int e1, int lane, int k, const int32_t *__restrict__ colidx,
+                                             const typename aval<T, AREAL>::type *__restrict__ vals,
+                                             const typename vec2<T>::type *__restrict__ ub, double (&sr)[CPL],
+                                             double (&si)[CPL])
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    constexpr int EPL = LPR >= NB ? 1 : NB / LPR;   // entries each lane fetches per batch
+    int mc[EPL];
+    AV mv[EPL];
+#pragma unroll
+    for (int u = 0; u < EPL; ++u) {
+        const int e = eb + lane + u * LPR;
+        const bool ok = (LPR <= NB || lane < NB) && e < e1;
+        mc[u] = ok ? colidx[e] : 0;
+        if (ok) mv[u] = vals[e];
+        else mv[u] = AV{};
+    }
+    const int cnt = (e1 - eb) < NB ? (e1 - eb) : NB;
+    int cs[NB];
+    AV as[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        cs[u] = __shfl(mc[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+        as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+    }
+    V xs[NB][CPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = lane + t * LPR;
+            xs[u][t] = ub[(int64_t)cs[u] * k + (j < k ? j : k - 1)];
+        }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        if (u < cnt) {
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                const double xr = (double)xs[u][t].x, xi = (double)xs[u][t].y;
+                if constexpr (AREAL) {
+                    sr[t] += as[u] * xr;
+                    si[t] += as[u] * xi;
+                } else {
+                    sr[t] += as[u].x * xr - as[u].y * xi;
+                    si[t] += as[u].x * xi + as[u].y * xr;
+                }
+            }
+        }
+    }
+}
+
+// s += A(row, :) * ub over the row's nonzeros e0 .. e1-1: full batches of NB,
+// then the remainder in one batch of NB, NB/2 or NB/4 -- the smallest that
+// holds it once it is at most half a batch -- so short tails issue few
+// padding gathers (11 nonzeros with NB = 8: 8 + 4 gathers, not 16).
 template <typename T, bool AREAL, int LPR, int CPL, int NB>
 __device__ __forceinline__ void row_gather(int e0, int e1, int lane, int k, const int32_t *__restrict__ colidx,
                                            const typename aval<T, AREAL>::type *__restrict__ vals,
                                            const typename vec2<T>::type *__restrict__ ub, double (&sr)[CPL],
                                            double (&si)[CPL])
 {
-    using V = typename vec2<T>::type;
-    using AV = typename aval<T, AREAL>::type;
-    constexpr int EPL = LPR >= NB ? 1 : NB / LPR;   // entries each lane fetches per batch
-    for (int eb = e0; eb < e1; eb += NB) {
-        int mc[EPL];
-        AV mv[EPL];
-#pragma unroll
-        for (int u = 0; u < EPL; ++u) {
-            const int e = eb + lane + u * LPR;
-            const bool ok = (LPR <= NB || lane < NB) && e < e1;
-            mc[u] = ok ? colidx[e] : 0;
-            if (ok) mv[u] = vals[e];
-            else mv[u] = AV{};
-        }
-        const int cnt = (e1 - eb) < NB ? (e1 - eb) : NB;
-        int cs[NB];
-        AV as[NB];
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            cs[u] = __shfl(mc[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
-            as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
-        }
-        // unconditional gathers, all in flight together: past the row's end
-        // cs[u] is 0 (a valid row) and the product is skipped below; lanes
-        // beyond k read column k-1 and their sums are never stored
-        V xs[NB][CPL];
-#pragma unroll
-        for (int u = 0; u < NB; ++u)
-#pragma unroll
-            for (int t = 0; t < CPL; ++t) {
-                const int j = lane + t * LPR;
-                xs[u][t] = ub[(int64_t)cs[u] * k + (j < k ? j : k - 1)];
-            }
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            if (u < cnt) {
-#pragma unroll
-                for (int t = 0; t < CPL; ++t) {
-                    const double xr = (double)xs[u][t].x, xi = (double)xs[u][t].y;
-                    if constexpr (AREAL) {
-                        sr[t] += as[u] * xr;
-                        si[t] += as[u] * xi;
-                    } else {
-                        sr[t] += as[u].x * xr - as[u].y * xi;
-                        si[t] += as[u].x * xi + as[u].y * xr;
-                    }
-                }
-            }
-        }
-    }
+    constexpr int NB2 = NB / 2 > 0 ? NB / 2 : 1, NB4 = NB / 4 > 0 ? NB / 4 : 1;
+    int eb = e0;
+    for (; e1 - eb >= NB; eb += NB) gather_batch<T, AREAL, LPR, CPL, NB>(eb, e1, lane, k, colidx, vals, ub, sr, si);
+    const int rem = e1 - eb;
+    if (rem > NB2) gather_batch<T, AREAL, LPR, CPL, NB>(eb, e1, lane, k, colidx, vals, ub, sr, si);
+    else if (rem > NB4) gather_batch<T, AREAL, LPR, CPL, NB2>(eb, e1, lane, k, colidx, vals, ub, sr, si);
+    else if (rem > 0) gather_batch<T, AREAL, LPR, CPL, NB4>(eb, e1, lane, k, colidx, vals, ub, sr, si);
 }
 
 // NB: nonzeros gathered per batch (16, or 8 for fewer registers and more
@@ -416,6 +436,143 @@ extern "C" int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage, const vo
     }
     set_error_msg("rsvdt_hip_rk4_stage: prec must be 64 or 32");
     return RSVDT_HIP_ERR_ARG;
+}
+
+// ---------------------------------------------------------------- fused small-operator stepping
+// For operators whose state fits in LDS (N x cw per workgroup), whole runs
+// of Horner steps in one launch: each workgroup owns cw columns of one block
+// (columns never interact in A*Y) and keeps y and the two stage vectors in
+// LDS, so a stage costs one barrier instead of one kernel launch.  The last
+// stage of step s adds g + s*gstride (the folded forcing sample; may be
+// null) and, when step i0+s stores a snapshot (SourceCode/SaveSnapshots.c:17-29),
+// writes y to snap[pos] in FP64.  Stage coefficients c[0..3] (h/4, h/3, h/2, h).
+template <typename T, bool AREAL>
+__global__ __launch_bounds__(256) void k_horner_fused(
+    int nrows, int k, int cw, int nsteps, int64_t bstride,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
+    typename vec2<T>::type *y, const typename vec2<T>::type *__restrict__ g, int64_t gstride,
+    double c1, double c2, double c3, double c4,
+    double2 *__restrict__ snap, int64_t snap_stride, int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    extern __shared__ double2 fsm[];
+    V *sy = (V *)fsm;
+    V *sa = sy + (int64_t)nrows * cw;
+    V *sb = sa + (int64_t)nrows * cw;
+    const int j0 = blockIdx.x * cw;
+    const int ncol = (k - j0) < cw ? (k - j0) : cw;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const AV *vals = (const AV *)vals_;
+    const int tot = nrows * cw;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / cw, c = e % cw;
+        V v;
+        v.x = 0; v.y = 0;
+        if (c < ncol) v = y[boff + (int64_t)r * k + j0 + c];
+        sy[e] = v;
+    }
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+        const int64_t i = i0 + st;
+        int64_t pos = -1;
+        if (snap != nullptr && i > Nt && (i - Nt) % RR == 0) {
+            pos = ((i - Nt) / RR - 1) % Nstore;
+            if (!dir) pos = Nstore - 1 - pos;
+        }
+        const V *gs = g != nullptr ? g + (int64_t)st * gstride + boff : nullptr;
+#pragma unroll 1
+        for (int stage = 0; stage < 4; ++stage) {
+            const V *vin = stage == 0 ? sy : (stage == 2 ? sb : sa);
+            V *out = stage == 3 ? sy : (stage == 1 ? sb : sa);
+            const double cs = stage == 0 ? c1 : (stage == 1 ? c2 : (stage == 2 ? c3 : c4));
+            for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+                const int r = e / cw, c = e % cw;
+                double sr = 0, si = 0;
+                const int e1 = rowptr[r + 1];
+                for (int p = rowptr[r]; p < e1; ++p) {
+                    const V x = vin[colidx[p] * cw + c];
+                    const double xr = (double)x.x, xi = (double)x.y;
+                    if constexpr (AREAL) {
+                        const double a = vals[p];
+                        sr += a * xr;
+                        si += a * xi;
+                    } else {
+                        const double2 a = vals[p];
+                        sr += a.x * xr - a.y * xi;
+                        si += a.x * xi + a.y * xr;
+                    }
+                }
+                const V yv = sy[e];
+                double ox = (double)yv.x + cs * sr, oy = (double)yv.y + cs * si;
+                if (stage == 3 && gs != nullptr && c < ncol) {
+                    const V gv = gs[(int64_t)r * k + j0 + c];
+                    ox += (double)gv.x;
+                    oy += (double)gv.y;
+                }
+                V o;
+                o.x = (T)ox;
+                o.y = (T)oy;
+                out[e] = o;       // stage 3 writes sy in place: each entry is read and written by one thread
+                if (stage == 3 && pos >= 0 && c < ncol)
+                    snap[pos * snap_stride + (int64_t)r * k + j0 + c] = make_double2((double)o.x, (double)o.y);
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / cw, c = e % cw;
+        if (c < ncol) y[boff + (int64_t)r * k + j0 + c] = sy[e];
+    }
+}
+
+static int fused_cw(int64_t nrows, int64_t k, int prec)
+{
+    const int64_t esz = prec == 64 ? 16 : 8;
+    int cw = 1;
+    while (cw * 2 <= k && cw < 8 && 3 * nrows * esz * (cw * 2) <= RSVDT_FUSED_LDS) cw *= 2;
+    return 3 * nrows * esz * cw <= RSVDT_FUSED_LDS ? cw : 0;
+}
+
+extern "C" int64_t rsvdt_hip_horner_fused_max_rows(int prec)
+{
+    return RSVDT_FUSED_LDS / (3 * (prec == 64 ? 16 : 8));
+}
+
+extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, void *y, const void *g, int64_t gstride,
+                                      const double *coef, int64_t nsteps, double *snap, int64_t snap_stride,
+                                      int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s)
+{
+    if (!d || d->nrows <= 0 || d->k <= 0 || d->nb <= 0 || d->nb > 65535 || !y || !coef ||
+        (snap && (RR <= 0 || Nstore <= 0)) || (d->prec != 64 && d->prec != 32)) {
+        set_error_msg("rsvdt_hip_horner_fused: bad arguments");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    const int cw = fused_cw(d->nrows, d->k, d->prec);
+    if (!cw) {
+        set_error_msg("rsvdt_hip_horner_fused: the operator is too large for the LDS-resident stepper");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    if (nsteps <= 0) return RSVDT_HIP_OK;
+    const dim3 grid((unsigned)((d->k + cw - 1) / cw), (unsigned)d->nb);
+    const size_t lds = (size_t)3 * (size_t)d->nrows * (size_t)cw * (d->prec == 64 ? 16 : 8);
+    const int64_t bs = d->nrows * d->k;
+    hipStream_t st = as_stream(s);
+#define RSVDT_FUSED_LAUNCH(T, AR)                                                                                 \
+    hipLaunchKernelGGL((k_horner_fused<T, AR>), grid, dim3(256), lds, st, (int)d->nrows, (int)d->k, cw,            \
+                       (int)nsteps, bs, d->rowptr, d->col, d->val, (typename vec2<T>::type *)y,                    \
+                       (const typename vec2<T>::type *)g, gstride, coef[0], coef[1], coef[2], coef[3],             \
+                       (double2 *)snap, snap_stride, i0, Nt, RR, Nstore, dir)
+    if (d->prec == 64) {
+        if (d->a_real) RSVDT_FUSED_LAUNCH(double, true);
+        else RSVDT_FUSED_LAUNCH(double, false);
+    } else {
+        if (d->a_real) RSVDT_FUSED_LAUNCH(float, true);
+        else RSVDT_FUSED_LAUNCH(float, false);
+    }
+#undef RSVDT_FUSED_LAUNCH
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_fused");
+    return RSVDT_HIP_OK;
 }
 
 static int check_desc(const rsvdt_spmm_desc *d, const char *who)

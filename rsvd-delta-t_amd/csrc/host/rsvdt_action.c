@@ -385,7 +385,9 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     if (max_steps > 0 && max_steps < rend) rend = max_steps;
     stepper st;
     RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
-    if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes) {
+    /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
+    const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && N <= rsvdt_hip_horner_fused_max_rows(prec);
+    if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
         RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2));
         desc_for(&st.desc, A, dir, prec, k, 1);
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
@@ -410,7 +412,7 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
     }
     /* graph replay of full chunks (form 0): one graph per buffer */
-    const int use_graph = g_rsvdt.opt.use_graph && st.form == 0;
+    const int use_graph = g_rsvdt.opt.use_graph && st.form == 0 && !fused;
     void *gexec[2] = {NULL, NULL};
     int *slots = NULL;
     if (use_graph) {
@@ -422,7 +424,14 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         const PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
         if (t_rk4) pt_begin(t_rk4);
-        if (use_graph && steps == st.S) {
+        if (fused) {
+            const double cf[4] = {st.h / 4, st.h / 3, st.h / 2, st.h};
+            int hr = rsvdt_hip_horner_fused(&st.desc, st.y, fslot(&st, buf, 0), (int64_t)N * k, cf, steps,
+                                            Yall ? Yall->d : NULL, (int64_t)N * k, i0, V->TS.Nt, V->TS.ResRatio,
+                                            Nstore, dir, g_rsvdt.stream);
+            if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
+            launches += 1;
+        } else if (use_graph && steps == st.S) {
             int *sl = slots + buf * st.S;
             if (Yall) {
                 int hr = rsvdt_hip_snap_slots(sl, st.S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream);
@@ -435,7 +444,7 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
             }
             launches += 4 * steps;
         }
-        for (PetscInt s = 0; s < steps && !(use_graph && steps == st.S); s++) {
+        for (PetscInt s = 0; s < steps && !fused && !(use_graph && steps == st.S); s++) {
             PetscInt i = i0 + s;
             double *snap = NULL;
             if (Yall && i > V->TS.Nt && (i - V->TS.Nt) % V->TS.ResRatio == 0) {
@@ -493,6 +502,7 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
     double *u2 = rsvdt_scratch(SCR_U2, blk * (size_t)G);
     double *acc = form ? rsvdt_scratch(SCR_ACC, blk * (size_t)G) : u2;
     if (!u1 || !u2 || !acc) return rsvdt_err(RSVDT_ERR_MEM, "TSExpADeltaT", "cannot allocate work buffers");
+    const int fused = form == 0 && g_rsvdt.opt.fuse_small && N <= rsvdt_hip_horner_fused_max_rows(64);
     for (PetscInt g0 = 0; g0 < nb; g0 += G) {
         PetscInt g = G < nb - g0 ? G : nb - g0;
         rsvdt_spmm_desc d;
@@ -500,6 +510,11 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         d.lanes = 0;
         double *y = X + 2 * (size_t)g0 * (size_t)N * (size_t)k;
         rsvdt_stream_t q = g_rsvdt.stream;
+        if (fused) {
+            const double cf[4] = {h / 4, h / 3, h / 2, h};
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, y, NULL, 0, cf, nsteps, NULL, 0, 0, 0, 1, 1, dir, q));
+            continue;
+        }
         for (PetscInt it = 0; it < nsteps; it++) {
             if (form == 0) {
                 RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_stage(&d, y, y, NULL, NULL, h / 4, u1, NULL, q));

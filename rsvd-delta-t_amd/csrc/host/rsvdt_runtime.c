@@ -200,6 +200,7 @@ static void default_options(RSVDT_Options *o)
     o->verbose = 0;
     o->overlap_forcing = 1;
     o->rk4_form = 0;
+    o->fuse_small = 1;
 }
 
 PetscErrorCode RSVDT_Initialize(int *argc, char ***argv)

@@ -163,6 +163,17 @@ class SketchNet:
             out.append((flat[a:a + o * i].reshape(o, i).astype(np.float64), flat[b:b + o].astype(np.float64)))
         return out
 
+    def save(self, path):
+        """FP32 weights of the trained network (numpy .npz, no pickled objects)."""
+        np.savez(path, w=self.w.get(np.float32, (self.nparam,)), layers=np.array(LAYERS, np.int64))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            if not np.array_equal(z["layers"], np.array(LAYERS, np.int64)) or z["w"].shape != (self.nparam,):
+                raise ValueError(f"{path}: weights do not match the network layout")
+            self.w.put(np.ascontiguousarray(z["w"], dtype=np.float32))
+        return self
+
     def grads_host(self):
         flat = self.g.get(np.float32, (self.nparam,))
         out = []
@@ -268,14 +279,15 @@ def _session(pkg, A, k, q, **opts):
     return pkg.Session(A, k=k, q=q, Nw=Nw, w=w, TL=FAMILY["TL"], dt=T / 4096, transient_removal=True, **opts)
 
 
-def rsvd_run(pkg, A, k, q, seed, learned=None):
+def rsvd_run(pkg, A, k, q, seed, learned=None, **opts):
     """Plain RSVD-Delta-t (main.c sequence) on A; learned: optional host array
-    (Nw_eff x N x ncol complex) of sketch columns.  Returns (S_hat (k x Nw_eff),
+    (Nw_eff x N x ncol complex) of sketch columns; opts: engine options
+    (precision, rk4_form, fuse_small, ...).  Returns (S_hat (k x Nw_eff),
     V_hat (N x k*Nw_eff), wall seconds)."""
     L = pkg.lib()
     _bind(L)
     t0 = time.time()
-    s = _session(pkg, A, k, q)
+    s = _session(pkg, A, k, q, **opts)
     if learned is not None:
         nwe, N, kl = learned.shape
         host = np.asfortranarray(learned.transpose(1, 0, 2).reshape(N, nwe * kl))
@@ -377,12 +389,55 @@ def evaluate(pkg, net, members, qs=(0, 1, 2), k=16, q_ref=4, seed=9001, tol=1e-6
     return rows, summary
 
 
+WEIGHTS = os.path.join(HERE, "learned_gl_sketch.npz")
+
+# the straightforward GPU form of the reference path: FP64 state, the
+# reference's RK4 stage order, one launch per stage, random sketch
+PLAIN = dict(precision=64, rk4_form=1, fuse_small=0)
+# the optimised form: Horner stages with the folded forcing, whole forcing
+# chunks per launch with the state in LDS, learned sketch columns
+FAST = dict(precision=64, rk4_form=0, fuse_small=1)
+
+
+def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242, fast=FAST, plain=PLAIN):
+    """Wall time to sigma_1 relative error <= tol (against a converged
+    reference: q_ref power iterations, optimised form) for the plain form
+    with a random sketch and for the optimised form with the learned sketch,
+    each with the fewest power iterations that reach tol.  Returns per-member
+    rows and the mean times."""
+    N = FAMILY["N"]
+    x = np.linspace(-85, 85, N)
+    rows = []
+    for i, (mu0, Ur) in enumerate(members):
+        A = family_operator(mu0, Ur, N)
+        Sref, _, _ = rsvd_run(pkg, A, k, q_ref, seed + 7919 * i, **fast)
+        om = omegas(FAMILY["Nw"], FAMILY["w"], False)[:Sref.shape[1]]
+        row = {"mu0": mu0, "Ur": Ur}
+        for name, opts, lrn in (("plain", plain, False), ("fast", fast, True)):
+            for q in range(qmax + 1):
+                t0 = time.time()
+                cols = net.columns(mu0, Ur, x, om) if lrn else None
+                S, _, _ = rsvd_run(pkg, A, k, q, seed + i, learned=cols, **opts)
+                wall = time.time() - t0
+                err = sigma1_error(S, Sref)
+                if err <= tol or q == qmax:
+                    row[name] = {"q": q, "wall_s": wall, "sigma1_err": err}
+                    break
+        rows.append(row)
+    ok = all(r["plain"]["sigma1_err"] <= tol and r["fast"]["sigma1_err"] <= tol for r in rows)
+    tp = float(np.mean([r["plain"]["wall_s"] for r in rows]))
+    tf = float(np.mean([r["fast"]["wall_s"] for r in rows]))
+    return rows, {"plain_s": tp, "fast_s": tf, "speedup": tp / tf, "all_reach_tol": ok,
+                  "plain_q": [r["plain"]["q"] for r in rows], "fast_q": [r["fast"]["q"] for r in rows]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=200)
     ap.add_argument("--eval", type=int, default=50)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--save", default=None, help="write the trained weights here (.npz)")
     args = ap.parse_args()
     pkg = _pkg()
     log = lambda s: print(s, flush=True)
@@ -396,6 +451,12 @@ def main():
     t0 = time.time()
     hist = net.train(data, epochs=args.epochs, log=log)
     t_train = time.time() - t0
+    if args.save:
+        net.save(args.save)
+    if not eval_m:
+        print(json.dumps({"learned_sketch": {"train_members": args.train, "epochs": args.epochs,
+                                             "final_loss": hist[-1], "train_s": round(t_train, 1)}}))
+        return
     rows, summary = evaluate(pkg, net, eval_m, qs=(0, 1, 2, 3), log=log)
     print(json.dumps({"learned_sketch": {"train_members": args.train, "eval_members": args.eval,
                                          "epochs": args.epochs, "final_loss": hist[-1],

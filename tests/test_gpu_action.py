@@ -24,10 +24,13 @@ CASES = [
 
 @pytest.mark.parametrize("N,Nw,cplx,tr,TL,k", CASES)
 @pytest.mark.parametrize("direct", [True, False])
-@pytest.mark.parametrize("form", [0, 1])
-def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form):
-    """form 0: Horner stages with the folded forcing; form 1: the reference's stage order."""
-    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct} form={form}", flush=True)
+@pytest.mark.parametrize("form,fuse", [(0, 1), (0, 0), (1, 0)])
+def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form, fuse):
+    """form 0: Horner stages with the folded forcing, whole chunks per launch
+    in LDS (fuse 1) or one launch per stage (fuse 0); form 1: the
+    reference's stage order."""
+    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct} form={form} fuse={fuse}",
+          flush=True)
     A = kat.gl_operator(N=N, L=20.0)
     if not cplx:
         A = A.real.astype(complex)
@@ -35,7 +38,7 @@ def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form):
     o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
     F = crandn(rng, N, k * o.Nw_eff)
     Yo = o.action(F, direct=direct)
-    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr, rk4_form=form)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr, rk4_form=form, fuse_small=fuse)
     assert (s.Ns, s.Nt, s.RR) == (o.Ns, o.Nt, o.RR)
     s.set_sketch(F)
     s.direct() if direct else s.adjoint()
@@ -55,7 +58,7 @@ def test_chunk_boundaries_and_lanes(pkg):
                                (7, 16, 1), (32, 32, 1), (5, 64, 1)]:
         print(f"chunk={chunk} lanes={lanes} form={form}", flush=True)
         s = pkg.Session(A, k=4, q=0, Nw=8, w=0.3, TL=1.3, dt=0.02, transient_removal=False, chunk_steps=chunk,
-                        spmm_lanes=lanes, rk4_form=form)
+                        spmm_lanes=lanes, rk4_form=form, fuse_small=0)
         s.set_sketch(F)
         s.direct()
         assert rel(s.get_Y(), Yo) < 1e-10
@@ -74,12 +77,13 @@ def test_config0_ginzburg_landau(pkg):
     assert o.Ns == 4096 and o.RR == 64
     F = crandn(rng, 500, k * o.Nw_eff)
     Yo = o.action(F)
-    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True)
-    s.set_sketch(F)
-    s.direct()
-    Yg = s.get_Y()
-    s.close()
-    assert rel(Yg, Yo) < 1e-10
+    for fuse in (1, 0):
+        s = pkg.Session(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, fuse_small=fuse)
+        s.set_sketch(F)
+        s.direct()
+        Yg = s.get_Y()
+        s.close()
+        assert rel(Yg, Yo) < 1e-10, fuse
 
 
 def test_fp32_state_form(pkg):
@@ -137,7 +141,7 @@ def test_graph_replay(pkg, tr, direct, prec):
     for g in (0, 1):
         print(f"graph={g} tr={tr} direct={direct} prec={prec}", flush=True)
         s = pkg.Session(A, k=3, q=1, Nw=16, w=0.3, TL=1.3, dt=0.02, transient_removal=tr, chunk_steps=7,
-                        use_graph=g, precision=prec)
+                        use_graph=g, precision=prec, fuse_small=0)
         s.set_sketch(F)
         s.direct() if direct else s.adjoint()
         Y1 = s.get_Y()

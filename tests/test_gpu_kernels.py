@@ -247,6 +247,64 @@ def test_horner_stage_xcd_remaps(L, pkg, k, lanes):
         assert np.array_equal(o, outs[0])
 
 
+@pytest.mark.parametrize("N,k,real,prec,nb,snaps", [(700, 10, False, 64, 1, True), (500, 16, True, 64, 1, True),
+                                                    (333, 3, False, 64, 4, False), (700, 10, False, 32, 1, True),
+                                                    (2048, 2, True, 64, 1, True), (90, 40, False, 32, 2, False)])
+def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
+    """Whole runs of Horner steps in one launch (LDS-resident state) against
+    numpy: y' = y + hA(y + h/2 A(y + h/3 A(y + h/4 Ay))) + g_s, with the FP64
+    snapshots of the steps that store one."""
+    print(f"fused N={N} k={k} real={real} prec={prec} nb={nb} snaps={snaps}", flush=True)
+    rng = np.random.default_rng(N + k)
+    A = sp.random(N, N, density=4.0 / N, random_state=k, format="csr", dtype=np.float64)
+    if not real:
+        A = A + 1j * sp.random(N, N, density=2.0 / N, random_state=k + 1, format="csr")
+    A = (A - 2 * sp.eye(N)).tocsr().astype(np.complex128)
+    A.sort_indices()
+    h, nsteps = 0.05, 23
+    i0, Nt, RR, Nstore, dr = 5, 9, 4, 3, 0
+    y0 = crandn(rng, nb, N, k)
+    g = crandn(rng, nsteps, N, k) if nb == 1 else None
+    cdt = np.complex128 if prec == 64 else np.complex64
+    y = y0.astype(cdt).astype(np.complex128)
+    snap_ref = np.zeros((Nstore, N, k), complex)
+    for st in range(nsteps):
+        for b in range(nb):
+            v = y[b]
+            for c in (h / 4, h / 3, h / 2):
+                v = (y[b] + c * (A @ v)).astype(cdt).astype(np.complex128) if prec == 32 else y[b] + c * (A @ v)
+            yn = y[b] + h * (A @ v) + (g[st].astype(cdt) if g is not None else 0)
+            y[b] = yn.astype(cdt).astype(np.complex128)
+        i = i0 + st
+        if snaps and i > Nt and (i - Nt) % RR == 0:
+            pos = ((i - Nt) // RR - 1) % Nstore
+            pos = Nstore - 1 - pos if not dr else pos
+            snap_ref[pos] = y[0]
+    vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
+    dy = Dev(pkg, y0.astype(cdt))
+    dg = Dev(pkg, g.astype(cdt)) if g is not None else None
+    dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
+    coef = np.array([h / 4, h / 3, h / 2, h])
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
+    L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), V, V, I64, V, I64, V, I64, I64, I64, I64, I64,
+                                         ctypes.c_int, V]
+    L.rsvdt_hip_horner_fused_max_rows.restype = I64
+    assert N <= L.rsvdt_hip_horner_fused_max_rows(prec)
+    rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), dy.p, dg.p if dg else None, N * k, coef.ctypes.data_as(V), nsteps,
+                                  dsnap.p if snaps else None, N * k, i0, Nt, RR, Nstore, dr, None)
+    assert rc == 0, L.rsvdt_hip_last_error()
+    out = dy.get(cdt, y.shape)
+    tol = 1e-12 if prec == 64 else 3e-5
+    assert rel(out, y) < tol
+    if snaps:
+        assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol
+    # an operator too large for the LDS-resident state is refused
+    big = Desc(L.rsvdt_hip_horner_fused_max_rows(prec) + 1, k, 1, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
+    assert L.rsvdt_hip_horner_fused(ctypes.byref(big), dy.p, None, 0, coef.ctypes.data_as(V), 1, None, 0, 0, 0, 1, 1,
+                                    1, None) != 0
+
+
 def test_horner_stage_rejects_bad_desc(L, pkg):
     d = Desc(0, 16, 1, None, None, None, 1, 64, 0, 1, 0)
     L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
