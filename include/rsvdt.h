@@ -240,6 +240,7 @@ typedef struct {
     int rk4_form;         /* 0: RK4 as y' = P(hA) y + g (Horner stages, forcing folded into one
                              precomputed term per step); 1: the reference's stage order */
     int fuse_small;       /* operators whose state fits in LDS: whole chunks of steps per launch */
+    int spmm_variant;     /* -1: autotuned; else force the stage-kernel variant (rsvdt_hip.h) */
 } RSVDT_Options;
 PetscErrorCode RSVDT_GetOptions(RSVDT_Options *o);
 PetscErrorCode RSVDT_SetOptions(const RSVDT_Options *o);

@@ -86,7 +86,19 @@ typedef struct {
     int variant;          /* 0: 16-nonzero gather batches; 1: + nontemporal row-local
                              streams; 2: 8-nonzero batches; 3: 8 + nontemporal; 4: 12-nonzero
                              batches; 5: 12 + nontemporal (variants apply when k <= lanes;
-                             4 and 5 only to the Horner stage with lanes >= 16) */
+                             4 and 5 only to the Horner stage with lanes >= 16); 6: the
+                             Horner stage on the row-grouped operator below (k <= lanes,
+                             16..64 lanes) */
+    /* row-grouped form of the same operator (variant 6): groups of `group`
+     * consecutive rows (2, 3, 4, 5 or 8); group q's distinct columns are
+     * gword[gptr[q] .. gptr[q+1]-1], each (column << 8 | mask of the group's
+     * rows that have an entry there), ascending by column; gval holds the
+     * values in (column, row) order from offset rowptr[q*group] on (real or
+     * complex FP64 like val).  Columns must be < 2^24. */
+    int group;
+    const int32_t *gptr;
+    const uint32_t *gword;
+    const void *gval;
 } rsvdt_spmm_desc;
 
 int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,

@@ -181,6 +181,58 @@ class Oracle:
             raise RuntimeError("orc_rsvdt_run failed")
         return U, V, S
 
+    # ---- weights and input/output matrices (numpy products around the C actions)
+    def _weights(self, Y, direct, before, W):
+        """ApplyWeightMats.c:16-80: direct before W_f^{-1/2} then B, after C then
+        W_q^{1/2}; adjoint before W_q^{1/2,H} then C^H, after B^H then W_f^{-1/2,H}."""
+        H = lambda M: M.conj().T
+        if direct:
+            seq = [("Wf_sqrt_inv", False), ("B", False)] if before else [("C", False), ("Wq_sqrt", False)]
+        else:
+            seq = [("Wq_sqrt", True), ("C", True)] if before else [("B", True), ("Wf_sqrt_inv", True)]
+        for name, herm in seq:
+            M = W.get(name)
+            if M is not None:
+                M = np.asarray(M.toarray() if hasattr(M, "toarray") else M, dtype=np.complex128)
+                Y = (H(M) if herm else M) @ Y
+        return np.asfortranarray(Y)
+
+    def _blocks(self, Y, fn):
+        k = self.k
+        out = np.array(Y, dtype=np.complex128, order="F", copy=True)
+        for m in range(self.Nw_eff):
+            out[:, m * k:(m + 1) * k] = fn(Y[:, m * k:(m + 1) * k])
+        return out
+
+    def _act(self, Y, W, direct):
+        Y = self._weights(Y, direct, True, W)
+        Y = self.action(Y, direct=direct)
+        return self._weights(Y, direct, False, W)
+
+    def run_weighted(self, F, **W):
+        """main.c:134-144 with ApplyWeightMats (W: any of B (N x Nb), C (Nc x N),
+        Wq_sqrt and Wq_sqrt_inv (Nc x Nc), Wf_sqrt_inv (Nb x Nb)): F is
+        Nb x k*Nw_eff; returns (U_hat Nc x k*Nw_eff, V_hat Nb x k*Nw_eff, S_hat).
+        QRAllFreqs / SVDAllFreqs* act per frequency block with orc_qr /
+        orc_svd_left; SVDAllFreqs.c:63-64 scales U by W_q^{-1/2} and V by W_f^{-1/2}."""
+        Y = self._act(np.asfortranarray(F, dtype=np.complex128), W, True)
+        for _ in range(self.c.q):
+            Y = self._act(self._blocks(Y, lambda b: qr(b)[0]), W, False)
+            Y = self._act(self._blocks(Y, lambda b: qr(b)[0]), W, True)
+        U = self._blocks(Y, lambda b: svd_left(b)[0])                  # SVDAllFreqsBeforeAdjoint, StoreU
+        Z = self._act(U, W, False)
+        S = np.zeros((self.k, self.Nw_eff))
+        V = np.array(Z, order="F", copy=True)
+        for m in range(self.Nw_eff):
+            Vm, sm = svd_left(Z[:, m * self.k:(m + 1) * self.k])
+            V[:, m * self.k:(m + 1) * self.k] = Vm
+            S[:, m] = sm
+        if W.get("Wq_sqrt_inv") is not None:
+            U = np.asarray(W["Wq_sqrt_inv"].toarray() if hasattr(W["Wq_sqrt_inv"], "toarray") else W["Wq_sqrt_inv"]) @ U
+        if W.get("Wf_sqrt_inv") is not None:
+            V = np.asarray(W["Wf_sqrt_inv"].toarray() if hasattr(W["Wf_sqrt_inv"], "toarray") else W["Wf_sqrt_inv"]) @ V
+        return U, V, S
+
     def time_steps(self, Fk, nvec, nsteps, direct=True):
         """Wall seconds of nvec x nsteps reference-structured RK4 steps (CPU baseline sample)."""
         Fk = np.asfortranarray(Fk, dtype=np.complex128)

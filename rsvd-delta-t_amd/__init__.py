@@ -91,7 +91,7 @@ class Directories(ctypes.Structure):
 class RSVDT_Options(ctypes.Structure):
     _fields_ = [("precision", _int), ("chunk_steps", _int), ("spmm_lanes", _int), ("xcd_remap", _int),
                 ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int), ("rk4_form", _int),
-                ("fuse_small", _int)]
+                ("fuse_small", _int), ("spmm_variant", _int)]
 
 
 class RSVDT_ActionStats(ctypes.Structure):
@@ -314,7 +314,7 @@ def get_options():
 
 # engine option defaults (RSVDT_Initialize's); a Session starts from these
 OPTION_DEFAULTS = dict(precision=64, chunk_steps=0, spmm_lanes=0, xcd_remap=1, use_graph=0, autotune=1, verbose=0,
-                       overlap_forcing=1, rk4_form=0, fuse_small=1)
+                       overlap_forcing=1, rk4_form=0, fuse_small=1, spmm_variant=-1)
 
 
 def set_options(**kw):
@@ -379,10 +379,31 @@ class Session:
         return (ctypes.byref(self.R), ctypes.byref(self.V), ctypes.byref(self.LNS), ctypes.byref(self.D),
                 ctypes.byref(self.W), ctypes.byref(self.dirs), ctypes.byref(self.TSR))
 
+    def set_weights(self, B=None, C=None, Wq_sqrt=None, Wq_sqrt_inv=None, Wf_sqrt_inv=None):
+        """ReadWeightMats (SourceCode/ReadWeightMats.c:26-99) with the matrices
+        held in memory (dense or scipy sparse): input matrix B (N x Nb), output
+        matrix C (Nc x N), output weight W_q^{1/2} and its inverse (Nc x Nc),
+        inverse input weight W_f^{-1/2} (Nb x Nb).  Sets Nb and Nc."""
+        import scipy.sparse as sp
+        W = self.W
+        for name, M, flag in (("B", B, "InputMatrixFlg"), ("C", C, "OutputMatrixFlg"),
+                              ("W_q_sqrt", Wq_sqrt, "OutputWeightFlg"), ("W_q_sqrt_inv", Wq_sqrt_inv, "InvOutputWeightFlg"),
+                              ("W_f_sqrt_inv", Wf_sqrt_inv, "InvInputWeightFlg")):
+            destroy(getattr(W, name))
+            setattr(W, name, None)
+            setattr(W, flag, 0)
+            if M is not None:
+                setattr(W, name, mat_from_csr(sp.csr_matrix(M, dtype=np.complex128)).value)
+                setattr(W, flag, 1)
+        self.V.RSVD.Nb = B.shape[1] if B is not None else self.N
+        self.V.RSVD.Nc = C.shape[0] if C is not None else self.N
+        self.LNS.RSVDt = self.V
+
     def set_sketch(self, F):
         F = np.asarray(F)
-        if F.shape != (self.N, self.k * self.Nw_eff):
-            raise ValueError(f"sketch must be {self.N} x {self.k * self.Nw_eff}")
+        Nb = self.V.RSVD.Nb or self.N
+        if F.shape != (Nb, self.k * self.Nw_eff):
+            raise ValueError(f"sketch must be {Nb} x {self.k * self.Nw_eff}")
         destroy(self.R.Y_hat)
         self.R.Y_hat = mat_from_dense(F, self.k).value
 
@@ -498,6 +519,9 @@ class Session:
             destroy(h)
         self.R.Y_hat = self.R.F_hat = self.Res.U_hat = self.Res.V_hat = self.Res.S_hat = None
         self.LNS.A = self.D.idft = self.D.dft = None
+        for name in ("B", "C", "W_q_sqrt", "W_q_sqrt_inv", "W_f_sqrt_inv"):
+            destroy(getattr(self.W, name))
+            setattr(self.W, name, None)
 
     def __del__(self):
         try:

@@ -374,10 +374,208 @@ static int launch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *
     return RSVDT_HIP_OK;
 }
 
+// ---------------------------------------------------------------- grouped gathers
+// The Horner stage on a row-grouped form of the operator: G consecutive rows
+// form a group and the group's distinct columns are listed once, each as a
+// word (column << 8 | mask of the group rows that use it), with the values in
+// (column, row) order starting at the group's first CSR offset.  Rows of
+// multi-field discretisations share the columns of their grid point, so a
+// group gathers each shared row of the stage vector once instead of once per
+// row (the 5-field 3D operator: 7 instead of 11 gathers per row).  LPR lanes
+// (one column each, k <= LPR) own a group; a batch of NB distinct columns is
+// fetched cooperatively, its values follow one coalesced load per lane, and
+// all NB gathers are in flight before the FMAs.
+template <typename T, bool AREAL, int LPR, int G, int NB>
+__device__ __forceinline__ void group_batch(int qb, int q1, int lane, int k, int jc, const uint32_t *__restrict__ gword,
+                                            const typename aval<T, AREAL>::type *__restrict__ gval, int &vbase,
+                                            const typename vec2<T>::type *__restrict__ ub, double (&sr)[G],
+                                            double (&si)[G])
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    static_assert(LPR >= NB, "one word per lane");
+    constexpr int VPL = (NB * G + LPR - 1) / LPR;     // values each lane fetches per batch
+    const int cnt = (q1 - qb) < NB ? (q1 - qb) : NB;
+    const uint32_t w = lane < cnt ? gword[qb + lane] : 0u;
+    const int pc = __popc(w & 0xffu);
+    int incl = pc;                                     // inclusive scan of the row counts
+#pragma unroll
+    for (int o = 1; o < NB; o <<= 1) {
+        const int t = __shfl_up(incl, o, LPR);
+        if (lane >= o) incl += t;
+    }
+    const int excl = incl - pc;
+    const int tot = __shfl(incl, NB - 1, LPR);
+    AV vv[VPL];
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+        const int e = lane + t * LPR;
+        if (e < tot) vv[t] = gval[vbase + e];
+        else vv[t] = AV{};
+    }
+    int cs[NB], os[NB];
+    uint32_t ms[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const uint32_t wu = __shfl(w, u, LPR);
+        cs[u] = (int)(wu >> 8);
+        ms[u] = wu & 0xffu;
+        os[u] = __shfl(excl, u, LPR);
+    }
+    V xs[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) xs[u] = ub[(int64_t)cs[u] * k + jc];   // padding words gather row 0
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        if (u >= cnt) continue;
+        const double xr = (double)xs[u].x, xi = (double)xs[u].y;
+        int idx = os[u];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            if (ms[u] & (1u << r)) {          // uniform over the group's lanes
+                AV src = vv[0];
+#pragma unroll
+                for (int t = 1; t < VPL; ++t)
+                    if (idx / LPR == t) src = vv[t];
+                const AV a = shfl_val(src, idx % LPR, LPR);
+                ++idx;
+                if constexpr (AREAL) {
+                    sr[r] += a * xr;
+                    si[r] += a * xi;
+                } else {
+                    sr[r] += a.x * xr - a.y * xi;
+                    si[r] += a.x * xi + a.y * xr;
+                }
+            }
+        }
+    }
+    vbase += tot;
+}
+
+template <typename T, bool AREAL, int LPR, int G>
+__global__ __launch_bounds__(256) void k_horner_group(
+    int nrows, int ngroups, int k, int xcd_remap, int64_t bstride, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ gptr, const uint32_t *__restrict__ gword, const void *__restrict__ gval_,
+    const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
+    const double2 *__restrict__ ys, const typename vec2<T>::type *__restrict__ g,
+    typename vec2<T>::type *out, double c, double2 *__restrict__ snap, const int *__restrict__ snap_slot,
+    int64_t snap_stride)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    constexpr int GPB = 256 / LPR, NB = 8;
+    const int bx = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
+    const int lane = threadIdx.x & (LPR - 1);
+    const int grp = bx * GPB + (int)(threadIdx.x / LPR);
+    if (grp >= ngroups) return;
+    const int r0 = grp * G;
+    const int jc = lane < k ? lane : k - 1;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const AV *gval = (const AV *)gval_;
+    if (snap_slot != nullptr) {
+        const int sl = *snap_slot;
+        snap = sl >= 0 ? snap + (int64_t)sl * snap_stride : nullptr;
+    }
+    const bool nty = (const void *)yv != (const void *)vin;
+    V yr[G], gr[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+        yr[r].x = 0; yr[r].y = 0; gr[r] = yr[r];
+        const int row = r0 + r;
+        if (row < nrows && lane < k) {
+            const int64_t idx = boff + (int64_t)row * k + lane;
+            yr[r] = ld_stream(yv + idx, nty);
+            if (g != nullptr) gr[r] = ld_stream(g + idx, true);
+        }
+    }
+    double sr[G], si[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) { sr[r] = 0; si[r] = 0; }
+    const int q0 = gptr[grp], q1 = gptr[grp + 1];
+    int vbase = rowptr[r0];
+    const V *ub = vin + boff;
+    int qb = q0;
+    for (; q1 - qb >= NB; qb += NB) group_batch<T, AREAL, LPR, G, NB>(qb, q1, lane, k, jc, gword, gval, vbase, ub, sr, si);
+    const int rem = q1 - qb;
+    if (rem > NB / 2) group_batch<T, AREAL, LPR, G, NB>(qb, q1, lane, k, jc, gword, gval, vbase, ub, sr, si);
+    else if (rem > NB / 4) group_batch<T, AREAL, LPR, G, NB / 2>(qb, q1, lane, k, jc, gword, gval, vbase, ub, sr, si);
+    else if (rem > 0) group_batch<T, AREAL, LPR, G, NB / 4>(qb, q1, lane, k, jc, gword, gval, vbase, ub, sr, si);
+    double ysr = 1.0, ysi = 0.0;
+    if (ys != nullptr) { ysr = ys[blockIdx.y].x; ysi = ys[blockIdx.y].y; }
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+        const int row = r0 + r;
+        if (row >= nrows || lane >= k) continue;
+        const int64_t idx = boff + (int64_t)row * k + lane;
+        double yx = (double)yr[r].x, yy = (double)yr[r].y;
+        if (ys != nullptr) {
+            const double tx = ysr * yx - ysi * yy;
+            yy = ysr * yy + ysi * yx;
+            yx = tx;
+        }
+        double ox = yx + c * sr[r], oy = yy + c * si[r];
+        if (g != nullptr) { ox += (double)gr[r].x; oy += (double)gr[r].y; }
+        V o;
+        o.x = (T)ox;
+        o.y = (T)oy;
+        out[idx] = o;
+        if (snap != nullptr) st_stream(snap + idx, make_double2((double)o.x, (double)o.y), true);
+    }
+}
+
+template <typename T, bool AREAL, int LPR, int G>
+static int launch_horner_group(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys,
+                               const void *g, double c, void *out, double *snap, const int *slot, int64_t sstride,
+                               hipStream_t s)
+{
+    using V = typename vec2<T>::type;
+    constexpr int GPB = 256 / LPR;
+    const int64_t ngroups = (d->nrows + G - 1) / G;
+    const unsigned gx = ceil_div(ngroups, GPB);
+    const unsigned gy = (unsigned)d->nb;
+    const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
+    hipLaunchKernelGGL((k_horner_group<T, AREAL, LPR, G>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows,
+                       (int)ngroups, (int)d->k, remap, d->nrows * d->k, d->rowptr, d->gptr,
+                       (const uint32_t *)d->gword, d->gval, (const V *)vin, (const V *)y, (const double2 *)ys,
+                       (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
+    return RSVDT_HIP_OK;
+}
+
+template <typename T, bool AREAL>
+static int dispatch_horner_group(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys,
+                                 const void *g, double c, void *out, double *snap, const int *slot, int64_t sstride,
+                                 hipStream_t s)
+{
+    int lpr = d->lanes;
+    if (lpr == 0) {
+        lpr = 16;
+        while (lpr < d->k && lpr < 64) lpr <<= 1;
+    }
+#define RSVDT_GCASE(L, GG)                                                                                       \
+    if (lpr == L && d->group == GG) return launch_horner_group<T, AREAL, L, GG>(d, vin, y, ys, g, c, out, snap,   \
+                                                                               slot, sstride, s);
+#define RSVDT_GCASES(L) RSVDT_GCASE(L, 2) RSVDT_GCASE(L, 3) RSVDT_GCASE(L, 4) RSVDT_GCASE(L, 5) RSVDT_GCASE(L, 8)
+    if (d->k <= lpr) {
+        RSVDT_GCASES(16) RSVDT_GCASES(32) RSVDT_GCASES(64)
+    }
+#undef RSVDT_GCASES
+#undef RSVDT_GCASE
+    set_error_msg("rsvdt_hip_horner_stage: grouped variant needs group in {2,3,4,5,8}, 16..64 lanes and k <= lanes");
+    return RSVDT_HIP_ERR_ARG;
+}
+
 template <typename T, bool AREAL>
 static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
                            double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
+    if (d->variant == 6) {
+        if (!d->group || !d->gptr || !d->gword || !d->gval) {
+            set_error_msg("rsvdt_hip_horner_stage: variant 6 needs the grouped operator arrays");
+            return RSVDT_HIP_ERR_ARG;
+        }
+        return dispatch_horner_group<T, AREAL>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+    }
     int lpr = d->lanes;
     const int k = (int)d->k;
     if (lpr == 0) {

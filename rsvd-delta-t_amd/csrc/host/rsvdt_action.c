@@ -71,6 +71,10 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->val = dir ? A->d_val64 : A->d_valH64;      /* FP64 values for both state precisions */
     d->a_real = A->is_real;
     d->prec = prec;
+    d->group = dir ? A->group : A->groupH;
+    d->gptr = dir ? A->d_gptr : A->d_gptrH;
+    d->gword = dir ? A->d_gword : A->d_gwordH;
+    d->gval = dir ? A->d_gval : A->d_gvalH;
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
     if (A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k && nb == 1) {
         d->lanes = A->tuned_lanes[f][p][q];
@@ -84,6 +88,8 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
         d->xcd_remap = g_rsvdt.opt.xcd_remap;
     }
     if (g_rsvdt.opt.spmm_lanes) d->lanes = g_rsvdt.opt.spmm_lanes;
+    if (g_rsvdt.opt.spmm_variant >= 0) d->variant = g_rsvdt.opt.spmm_variant;
+    if (d->variant == 6 && (!d->group || g_rsvdt.opt.rk4_form)) d->variant = 3;   /* no grouped form */
 }
 
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
@@ -325,7 +331,9 @@ static void pt_free(phase_timer *t)
 static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
 {
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
-    if ((A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k) || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes) return 0;
+    if ((A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k) || !g_rsvdt.opt.autotune || g_rsvdt.opt.spmm_lanes ||
+        g_rsvdt.opt.spmm_variant >= 0)
+        return 0;
     rsvdt_spmm_desc dsc;
     desc_for(&dsc, A, dir, prec, k, 1);
     void *e0, *e1;
@@ -337,7 +345,7 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         if ((k + l - 1) / l > 4) continue;            /* at most 4 columns per lane */
         if (l >= 2 * k && l > 4) continue;            /* no more than half the lanes idle */
         const int cpl = (int)((k + l - 1) / l);
-        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? 6 : 4);
+        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? (dsc.group ? 7 : 6) : 4);
         for (int v = 0; v < nvar; v++)
             for (int r = 0; r < 4; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;

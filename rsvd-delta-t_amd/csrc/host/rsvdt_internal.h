@@ -36,6 +36,12 @@ struct _p_RSVDT_Mat {
     double *h_val;           /* 2*nnz */
     int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
     void *d_val64, *d_valH64;
+    /* row-grouped form of A and A^H for the grouped-gather stage kernel
+     * (group = 0: not built, the grouping would not save gathers) */
+    int group, groupH;
+    int32_t *d_gptr, *d_gptrH;
+    uint32_t *d_gword, *d_gwordH;
+    void *d_gval, *d_gvalH;
     /* autotuned SpMM launch shape per [stepping form][precision 64/32]
      * [direct/adjoint]: lanes per row, kernel variant, XCD remap; tuned_ok
      * marks a tuned entry, valid for blocks of tuned_k columns */

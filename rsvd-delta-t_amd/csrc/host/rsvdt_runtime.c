@@ -201,6 +201,7 @@ static void default_options(RSVDT_Options *o)
     o->overlap_forcing = 1;
     o->rk4_form = 0;
     o->fuse_small = 1;
+    o->spmm_variant = -1;
 }
 
 PetscErrorCode RSVDT_Initialize(int *argc, char ***argv)
@@ -429,6 +430,8 @@ PetscErrorCode RSVDT_MatDestroy(Mat *pm)
     rsvdt_hip_free(m->d_rowptrH); rsvdt_hip_free(m->d_colH);
     rsvdt_hip_free(m->d_val64);
     rsvdt_hip_free(m->d_valH64);
+    rsvdt_hip_free(m->d_gptr); rsvdt_hip_free(m->d_gword); rsvdt_hip_free(m->d_gval);
+    rsvdt_hip_free(m->d_gptrH); rsvdt_hip_free(m->d_gwordH); rsvdt_hip_free(m->d_gvalH);
     free(m->h_rowptr); free(m->h_col); free(m->h_val);
     free(m);
     *pm = NULL;
@@ -507,6 +510,91 @@ static PetscErrorCode upload_csr(PetscInt rows, const int64_t *rp, const int64_t
     return 0;
 }
 
+/* ---- row-grouped form (for the grouped-gather stage kernel) ----------- */
+typedef struct { int64_t col; int r; int64_t e; } gent;
+static int gent_cmp(const void *a, const void *b)
+{
+    const gent *x = a, *y = b;
+    if (x->col != y->col) return x->col < y->col ? -1 : 1;
+    return x->r - y->r;
+}
+
+/* Distinct columns of the groups of G rows among the first `rows` rows. */
+static int64_t group_unique(PetscInt rows, const int64_t *rp, const int64_t *ci, int G, gent *buf)
+{
+    int64_t u = 0;
+    for (PetscInt r0 = 0; r0 < rows; r0 += G) {
+        int n = 0;
+        for (int r = 0; r < G && r0 + r < rows; r++)
+            for (int64_t e = rp[r0 + r]; e < rp[r0 + r + 1]; e++) buf[n++] = (gent){ci[e], r, e};
+        qsort(buf, (size_t)n, sizeof(gent), gent_cmp);
+        for (int i = 0; i < n; i++) u += (i == 0 || buf[i].col != buf[i - 1].col);
+    }
+    return u;
+}
+
+/* Build and upload the row-grouped form of a CSR matrix: the group size in
+ * {2, 3, 4, 5, 8} with the fewest distinct columns per row on a sample of
+ * the rows, kept only when it saves at least 15 % of the gathers. */
+static PetscErrorCode upload_grouped(PetscInt rows, PetscInt cols, const int64_t *rp, const int64_t *ci, const double *v,
+                                     int is_real, int *G_out, int32_t **d_gptr, uint32_t **d_gword, void **d_gval)
+{
+    *G_out = 0;
+    const PetscInt nnz = rp[rows];
+    if (rows < 64 || cols >= (1LL << 24) || nnz > 0x7fffffffLL) return 0;
+    int64_t maxrow = 0;
+    for (PetscInt r = 0; r < rows; r++) if (rp[r + 1] - rp[r] > maxrow) maxrow = rp[r + 1] - rp[r];
+    if (maxrow > 64) return 0;
+    gent *buf = malloc(sizeof(gent) * (size_t)(8 * maxrow + 1));
+    if (!buf) return rsvdt_err(RSVDT_ERR_MEM, "MatLoad", "out of memory");
+    const int cand[5] = {2, 3, 4, 5, 8};
+    const PetscInt ns = rows < 65520 ? rows : 65520;    /* sample: a multiple of 2, 3, 4, 5, 8 */
+    const double base = (double)(rp[ns] - rp[0]);
+    int best = 0;
+    double bu = base;
+    for (int c = 0; c < 5; c++) {
+        double u = (double)group_unique(ns, rp, ci, cand[c], buf);
+        if (u < bu) { bu = u; best = cand[c]; }
+    }
+    if (!best || bu > 0.85 * base) { free(buf); return 0; }
+    const int G = best;
+    const int64_t ng = (rows + G - 1) / G;
+    int32_t *gptr = malloc(sizeof(int32_t) * (size_t)(ng + 1));
+    uint32_t *gword = malloc(sizeof(uint32_t) * (size_t)(nnz ? nnz : 1));
+    const size_t nv = (size_t)(nnz ? nnz : 1) * (is_real ? 1 : 2);
+    double *gval = malloc(sizeof(double) * nv);
+    if (!gptr || !gword || !gval) { free(buf); free(gptr); free(gword); free(gval); return rsvdt_err(RSVDT_ERR_MEM, "MatLoad", "out of memory"); }
+    int64_t q = 0;
+    for (int64_t g = 0; g < ng; g++) {
+        const PetscInt r0 = g * G;
+        gptr[g] = (int32_t)q;
+        int n = 0;
+        for (int r = 0; r < G && r0 + r < rows; r++)
+            for (int64_t e = rp[r0 + r]; e < rp[r0 + r + 1]; e++) buf[n++] = (gent){ci[e], r, e};
+        qsort(buf, (size_t)n, sizeof(gent), gent_cmp);
+        int64_t vo = rp[r0];
+        for (int i = 0; i < n; i++) {
+            if (i == 0 || buf[i].col != buf[i - 1].col) gword[q++] = ((uint32_t)buf[i].col << 8);
+            gword[q - 1] |= 1u << buf[i].r;
+            if (is_real) gval[vo] = v[2 * buf[i].e];
+            else { gval[2 * vo] = v[2 * buf[i].e]; gval[2 * vo + 1] = v[2 * buf[i].e + 1]; }
+            vo++;
+        }
+    }
+    gptr[ng] = (int32_t)q;
+    free(buf);
+    int rc = rsvdt_hip_malloc((void **)d_gptr, sizeof(int32_t) * (size_t)(ng + 1));
+    if (!rc) rc = rsvdt_hip_malloc((void **)d_gword, sizeof(uint32_t) * (size_t)(q ? q : 1));
+    if (!rc) rc = rsvdt_hip_malloc(d_gval, sizeof(double) * nv);
+    if (!rc) rc = rsvdt_hip_memcpy_h2d(*d_gptr, gptr, sizeof(int32_t) * (size_t)(ng + 1));
+    if (!rc && q) rc = rsvdt_hip_memcpy_h2d(*d_gword, gword, sizeof(uint32_t) * (size_t)q);
+    if (!rc) rc = rsvdt_hip_memcpy_h2d(*d_gval, gval, sizeof(double) * nv);
+    free(gptr); free(gword); free(gval);
+    if (rc) return rsvdt_err(rc, "MatLoad", "%s", rsvdt_hip_last_error());
+    *G_out = G;
+    return 0;
+}
+
 /* Upload A and A^H (built on the host by a counting transpose).  Called
  * lazily on first device use, so that loading and validating matrices needs
  * no GPU. */
@@ -518,6 +606,8 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     RSVDT_CHK(rsvdt_aij_real_check(m));
     RSVDT_CHK(upload_csr(m->rows, m->h_rowptr, m->h_col, m->h_val, m->nnz, m->is_real, &m->d_rowptr, &m->d_col,
                          &m->d_val64));
+    RSVDT_CHK(upload_grouped(m->rows, m->cols, m->h_rowptr, m->h_col, m->h_val, m->is_real, &m->group, &m->d_gptr,
+                             &m->d_gword, &m->d_gval));
     PetscInt nr = m->cols, nnz = m->nnz;
     int64_t *rp = calloc((size_t)nr + 1, sizeof(int64_t));
     int64_t *ci = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
@@ -536,6 +626,8 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
             v[2 * p + 1] = -m->h_val[2 * e + 1];
         }
     PetscErrorCode ierr = upload_csr(nr, rp, ci, v, nnz, m->is_real, &m->d_rowptrH, &m->d_colH, &m->d_valH64);
+    if (!ierr) ierr = upload_grouped(nr, m->rows, rp, ci, v, m->is_real, &m->groupH, &m->d_gptrH, &m->d_gwordH,
+                                     &m->d_gvalH);
     free(rp); free(ci); free(v); free(fill);
     memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
     return ierr;

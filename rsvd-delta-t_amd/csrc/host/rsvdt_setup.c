@@ -142,6 +142,7 @@ PetscErrorCode ReadUserInput(RSVDt_vars *RSVDt, Weight_matrices *W, LNS_vars *LN
     { int ok; PetscInt v; if (rsvdt_opt_get_int("SpMMLanes", &v, &ok) && ok) o.spmm_lanes = (int)v; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("UseGraph", &b, &ok) && ok) o.use_graph = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("FuseSmall", &b, &ok) && ok) o.fuse_small = b; }
+    { int ok; PetscInt v; if (rsvdt_opt_get_int("SpMMVariant", &v, &ok) && ok) o.spmm_variant = (int)v; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Autotune", &b, &ok) && ok) o.autotune = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("XCDRemap", &b, &ok) && ok) o.xcd_remap = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Verbose", &b, &ok) && ok) o.verbose = b; }

@@ -37,3 +37,38 @@ def init(pkg):
     rc = L.rsvdt_hip_init(0)
     assert rc == 0, L.rsvdt_hip_last_error()
     return L
+
+
+V_ = ctypes.c_void_p
+I64_ = ctypes.c_int64
+
+
+class Desc(ctypes.Structure):
+    """Mirror of rsvdt_spmm_desc (include/rsvdt_hip.h)."""
+    _fields_ = [("nrows", I64_), ("k", I64_), ("nb", I64_), ("rowptr", V_), ("col", V_), ("val", V_),
+                ("a_real", ctypes.c_int), ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int),
+                ("variant", ctypes.c_int), ("group", ctypes.c_int), ("gptr", V_), ("gword", V_), ("gval", V_)]
+
+
+def grouped_form(A, G):
+    """Row-grouped form of a CSR matrix (rsvdt_spmm_desc.group/gptr/gword/gval):
+    per group of G rows its distinct columns, ascending, as (col << 8 | row mask),
+    and the values in (column, row) order -- built here independently of the
+    library's own builder."""
+    A = A.tocsr()
+    A.sort_indices()
+    N = A.shape[0]
+    rows = np.repeat(np.arange(N), np.diff(A.indptr))
+    grp, r = rows // G, rows % G
+    order = np.lexsort((r, A.indices, grp))
+    g_s, c_s, r_s = grp[order], A.indices[order].astype(np.int64), r[order]
+    vals = A.data[order]
+    new = np.ones(len(order), bool)
+    new[1:] = (g_s[1:] != g_s[:-1]) | (c_s[1:] != c_s[:-1])
+    starts = np.flatnonzero(new)
+    words = (c_s[starts] << 8) | np.bitwise_or.reduceat(np.left_shift(1, r_s), starts)
+    ng = (N + G - 1) // G
+    counts = np.bincount(g_s[starts], minlength=ng)
+    gptr = np.zeros(ng + 1, np.int32)
+    np.cumsum(counts, out=gptr[1:])
+    return gptr, words.astype(np.uint32), vals

@@ -24,13 +24,13 @@ CASES = [
 
 @pytest.mark.parametrize("N,Nw,cplx,tr,TL,k", CASES)
 @pytest.mark.parametrize("direct", [True, False])
-@pytest.mark.parametrize("form,fuse", [(0, 1), (0, 0), (1, 0)])
-def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form, fuse):
+@pytest.mark.parametrize("form,fuse,variant", [(0, 1, -1), (0, 0, -1), (0, 0, 6), (1, 0, -1)])
+def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form, fuse, variant):
     """form 0: Horner stages with the folded forcing, whole chunks per launch
-    in LDS (fuse 1) or one launch per stage (fuse 0); form 1: the
-    reference's stage order."""
-    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct} form={form} fuse={fuse}",
-          flush=True)
+    in LDS (fuse 1) or one launch per stage (fuse 0; variant 6: on the
+    row-grouped operator); form 1: the reference's stage order."""
+    print(f"action N={N} Nw={Nw} complex={cplx} TR={tr} TL={TL} k={k} direct={direct} form={form} fuse={fuse} "
+          f"variant={variant}", flush=True)
     A = kat.gl_operator(N=N, L=20.0)
     if not cplx:
         A = A.real.astype(complex)
@@ -38,7 +38,8 @@ def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form, fuse):
     o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr)
     F = crandn(rng, N, k * o.Nw_eff)
     Yo = o.action(F, direct=direct)
-    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr, rk4_form=form, fuse_small=fuse)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=TL, dt=0.02, transient_removal=tr, rk4_form=form, fuse_small=fuse,
+                    spmm_variant=variant)
     assert (s.Ns, s.Nt, s.RR) == (o.Ns, o.Nt, o.RR)
     s.set_sketch(F)
     s.direct() if direct else s.adjoint()

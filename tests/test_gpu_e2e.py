@@ -89,3 +89,54 @@ TransICDir: none
     assert U0.shape == (N, k)
     assert abs(abs(np.vdot(U0[:, 0], Uo[:, 0])) - 1) < 1e-9
     assert (out / "vars.yaml").read_text() == yaml
+
+
+def test_cli_transient_run(pkg, tmp_path):
+    """rsvdt -inputs variables.yaml with TransRun (SourceCode/main.c:119-124):
+    the initial vector read from TransICDir, norm monitor every TransSaveMod
+    steps, saved snapshots and the removal estimate of each period, against the
+    oracle; the outputs land in TransientSnapshots_0 with the inputs copy."""
+    N, Nw = 60, 8
+    A = kat.gl_operator(N=N, L=20.0)
+    rng = np.random.default_rng(12)
+    q0 = rng.random(N) + 1j * rng.random(N)
+    petsc_bin.write_mat(str(tmp_path / "A"), A)
+    (tmp_path / "res").mkdir()
+    petsc_bin.write_vec(str(tmp_path / "res" / "q0"), q0)
+    yaml = f"""RootDir: {tmp_path}/
+ResultsDir: res/
+OperatorDir: A
+k: 2
+q: 0
+Nw: {Nw}
+w: 0.3
+TransientLength: 2
+dt: 0.02
+TransRun: true
+TransICFlg: true
+TransICDir: q0
+TransSave: true
+TransSaveMod: 200
+TransPeriods: 2
+TransRemovalEst: true
+TransConVal: 1e-30
+"""
+    (tmp_path / "vars.yaml").write_text(yaml)
+    r = subprocess.run([pkg.BINPATH, "-inputs", "vars.yaml"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-3000:], r.stderr)
+    assert r.returncode == 0
+    out = tmp_path / "res" / "TransientSnapshots_0"
+    o = oracle.Oracle(A, k=2, q=0, Nw=Nw, w=0.3, TL=2.0, dt=0.02, transient_removal=False)
+    ref = o.trans_run(q0, periods=2, save_mod=200, con_val=1e-30, est=True, save=True)
+    norms = petsc_bin.read_vec(str(out / "q_transient_norms")).real
+    assert np.max(np.abs(norms - ref["norms"]) / ref["norms"]) < 1e-10
+    last = petsc_bin.read_vec(str(out / "q_transient_last_snapshot"))
+    assert np.linalg.norm(last - ref["last"]) / np.linalg.norm(ref["last"]) < 1e-10
+    for pos in range(ref["Nt_saved"]):
+        assert (out / f"q_transient_{pos}").exists()
+    for p in (1, 2):
+        upd = petsc_bin.read_vec(str(out / f"Updated_transient_norm_period_{p}")).real
+        assert np.allclose(upd, ref["upd"][p - 1], rtol=1e-8, atol=1e-12)
+    assert "Total integration time" in r.stdout and "DONE" in r.stdout
+    assert (out / "vars.yaml").read_text() == yaml

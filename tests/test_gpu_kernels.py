@@ -8,7 +8,7 @@ import scipy.sparse as sp
 
 import kat
 from conftest import crandn, rel
-from gpu_util import Dev, init
+from gpu_util import Desc, Dev, grouped_form, init
 
 pytestmark = pytest.mark.gpu
 V = ctypes.c_void_p
@@ -136,9 +136,6 @@ def rk4_ref(A, y, F0, Fh, F1, h):
     return y + h / 6 * k1 + h / 3 * k2 + h / 3 * k3 + h / 6 * k4
 
 
-class Desc(ctypes.Structure):
-    _fields_ = [("nrows", I64), ("k", I64), ("nb", I64), ("rowptr", V), ("col", V), ("val", V), ("a_real", ctypes.c_int),
-                ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
 @pytest.mark.parametrize("k,real,prec,nb,lanes,variant", [
@@ -303,6 +300,57 @@ def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
     big = Desc(L.rsvdt_hip_horner_fused_max_rows(prec) + 1, k, 1, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
     assert L.rsvdt_hip_horner_fused(ctypes.byref(big), dy.p, None, 0, coef.ctypes.data_as(V), 1, None, 0, 0, 0, 1, 1,
                                     1, None) != 0
+
+
+def _point_operator(P, F, rng, cplx):
+    """F fields per grid point: dense F x F coupling at the point plus a
+    diagonal (same-field) coupling to the periodic neighbours p+-1, p+-7, and a
+    few random entries; rows of one point share their columns."""
+    S = sp.diags([1.0, 1.0, 1.0, 1.0], [-7, -1, 1, 7], shape=(P, P), format="csr")
+    S = (S + sp.diags([1.0, 1.0, 1.0, 1.0], [P - 1, P - 7, -(P - 1), -(P - 7)], shape=(P, P))).tocsr()
+    B = rng.standard_normal((F, F)) + (1j * rng.standard_normal((F, F)) if cplx else 0)
+    A = sp.kron(sp.eye(P), B) + sp.kron(S, sp.diags(rng.standard_normal(F)))
+    R = sp.random(P * F, P * F, density=0.5 / (P * F), random_state=int(rng.integers(1000)), format="csr")
+    return (A + R - 2 * sp.eye(P * F)).tocsr().astype(np.complex128)
+
+
+@pytest.mark.parametrize("F,G,k,real,prec,nb,scaled", [(5, 5, 32, True, 64, 1, False), (3, 3, 16, True, 64, 2, True),
+                                                       (5, 8, 32, False, 64, 1, True), (4, 4, 10, False, 32, 1, False),
+                                                       (2, 2, 64, True, 32, 3, False), (3, 5, 16, False, 64, 1, True)])
+def test_horner_grouped(L, pkg, F, G, k, real, prec, nb, scaled):
+    """The Horner stage on the row-grouped operator (variant 6) against numpy,
+    with the forcing term, block scales and the snapshot copy."""
+    print(f"grouped F={F} G={G} k={k} real={real} prec={prec} nb={nb}", flush=True)
+    rng = np.random.default_rng(F * 10 + G + k)
+    A = _point_operator(811, F, rng, not real)
+    if real:
+        A = A.real.astype(np.complex128)
+    N = A.shape[0]
+    A.sort_indices()
+    c = 0.021
+    vin, y, g = crandn(rng, nb, N, k), crandn(rng, nb, N, k), crandn(rng, nb, N, k)
+    ys = crandn(rng, nb) if scaled else np.ones(nb, complex)
+    cdt = np.complex128 if prec == 64 else np.complex64
+    vin, y, g = vin.astype(cdt), y.astype(cdt), g.astype(cdt)
+    ref = np.stack([ys[b] * y[b] + c * (A @ vin[b].astype(complex)) + g[b] for b in range(nb)])
+    gptr, gword, gval = grouped_form(A, G)
+    vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)
+    gv = gval.real.astype(np.float64) if real else gval.astype(np.complex128)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
+    dgp, dgw, dgv = Dev(pkg, gptr), Dev(pkg, gword), Dev(pkg, gv)
+    dvin, dy, dg, dys = Dev(pkg, vin), Dev(pkg, y), Dev(pkg, g), Dev(pkg, ys.astype(np.complex128))
+    dout, dsnap = Dev(pkg, nbytes=y.nbytes), Dev(pkg, nbytes=nb * N * k * 16)
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 6, G, dgp.p, dgw.p, dgv.p)
+    L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
+    rc = L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, dys.p if scaled else None, dg.p, c, dout.p,
+                                  dsnap.p, None)
+    assert rc == 0, L.rsvdt_hip_last_error()
+    out = dout.get(cdt, y.shape)
+    assert rel(out, ref) < (1e-14 if prec == 64 else 2e-6)
+    assert np.array_equal(dsnap.get(np.complex128, y.shape), out.astype(np.complex128))
+    # variant 6 without the grouped arrays is refused
+    d0 = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 6, 0, None, None, None)
+    assert L.rsvdt_hip_horner_stage(ctypes.byref(d0), dvin.p, dy.p, None, None, c, dout.p, None, None) != 0
 
 
 def test_horner_stage_rejects_bad_desc(L, pkg):

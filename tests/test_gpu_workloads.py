@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import crandn, rel
-from gpu_util import Dev, init
+from gpu_util import Desc, Dev, grouped_form, init
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rsvd-delta-t_amd"))
 import workloads  # noqa: E402
@@ -21,9 +21,6 @@ V = ctypes.c_void_p
 I64 = ctypes.c_int64
 
 
-class Desc(ctypes.Structure):
-    _fields_ = [("nrows", I64), ("k", I64), ("nb", I64), ("rowptr", V), ("col", V), ("val", V), ("a_real", ctypes.c_int),
-                ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +40,7 @@ def operator(name):
 
 
 @pytest.mark.parametrize("name", ["flow2d", "flow3d"])
-@pytest.mark.parametrize("prec,variant,remap", [(64, 3, 1), (64, 0, 0), (32, 2, 1)])
+@pytest.mark.parametrize("prec,variant,remap", [(64, 3, 1), (64, 0, 0), (32, 2, 1), (64, 6, 0), (32, 6, 3)])
 def test_horner_stage_full_size(L, pkg, name, prec, variant, remap):
     print(f"horner full size {name} prec={prec} variant={variant} remap={remap}", flush=True)
     A, k = operator(name)
@@ -58,6 +55,11 @@ def test_horner_stage_full_size(L, pkg, name, prec, variant, remap):
     dv = Dev(pkg, A.data.real.astype(np.float64))
     dvin, dy, dg, dout = Dev(pkg, vin), Dev(pkg, y), Dev(pkg, g), Dev(pkg, nbytes=y.nbytes)
     d = Desc(N, k, 1, drp.p, dci.p, dv.p, 1, prec, 0, remap, variant)
+    if variant == 6:    # the row-grouped form, one group per grid point (3 or 5 fields)
+        G = 3 if name == "flow2d" else 5
+        gptr, gword, gval = grouped_form(A, G)
+        dgp, dgw, dgv = Dev(pkg, gptr), Dev(pkg, gword), Dev(pkg, gval.real.astype(np.float64))
+        d.group, d.gptr, d.gword, d.gval = G, dgp.p, dgw.p, dgv.p
     L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
     assert L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, None, dg.p, c, dout.p, None, None) == 0
     out = dout.get(cdt, y.shape)
