@@ -129,19 +129,20 @@ int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *vin, const v
 /* Plain block SpMM: out_b = M * in_b for nb blocks of k columns (complex FP64
  * values, int32 CSR).  in blocks are ncols x k (stride in_bstride), out blocks
  * nrows x k (stride out_bstride).  Used for the B, C and weight matrices.   */
-/* Whole runs of Horner steps for operators whose state fits in LDS
- * (nrows <= rsvdt_hip_horner_fused_max_rows(prec)): y (d->nb blocks of
- * N x k, FP64 or FP32 per d->prec) is advanced by nsteps steps in place;
- * coef = {h/4, h/3, h/2, h}; step s adds the forcing sample g + s*gstride
- * (N x k, state precision; g may be NULL) at its last stage and, when step
- * i0+s stores a snapshot (i > Nt, (i-Nt) % RR == 0, column (i-Nt)/RR-1 mod
- * Nstore, reversed when dir == 0), writes y in FP64 to snap + pos*snap_stride
+/* Whole runs of Horner steps for operators that fit in LDS with the state of
+ * one column (rsvdt_hip_horner_fused_fits: the CSR matrix of nnz nonzeros,
+ * y and two stage vectors within 160 KB): y (d->nb blocks of N x k, FP64 or
+ * FP32 per d->prec) is advanced by nsteps steps in place; coef = {h/4, h/3,
+ * h/2, h}; step s adds the forcing sample g + s*gstride (N x k, state
+ * precision; g may be NULL) at its last stage and, when step i0+s stores a
+ * snapshot (i > Nt, (i-Nt) % RR == 0, column (i-Nt)/RR-1 mod Nstore,
+ * reversed when dir == 0), writes y in FP64 to snap + pos*snap_stride
  * (complex entries; snap may be NULL).  One launch, one workgroup per
- * (column group, block). */
-int64_t rsvdt_hip_horner_fused_max_rows(int prec);
-int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, void *y, const void *g, int64_t gstride, const double *coef,
-                           int64_t nsteps, double *snap, int64_t snap_stride, int64_t i0, int64_t Nt, int64_t RR,
-                           int64_t Nstore, int dir, rsvdt_stream_t s);
+ * (column, block). */
+int rsvdt_hip_horner_fused_fits(int64_t nrows, int64_t nnz, int a_real, int prec);
+int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, void *y, const void *g, int64_t gstride,
+                           const double *coef, int64_t nsteps, double *snap, int64_t snap_stride, int64_t i0,
+                           int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s);
 int rsvdt_hip_spmm_blocks(int64_t nrows, int64_t k, int64_t nb, const int32_t *rowptr, const int32_t *col,
                           const double *val_complex, const double *in, int64_t in_bstride,
                           double *out, int64_t out_bstride, rsvdt_stream_t s);

@@ -119,7 +119,7 @@ HIP_ENTRY_POINTS = [
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
-    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_fused", "rsvdt_hip_horner_fused_max_rows", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
+    "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_fused", "rsvdt_hip_horner_fused_fits", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
     "rsvdt_hip_graph_begin", "rsvdt_hip_graph_end", "rsvdt_hip_graph_launch", "rsvdt_hip_graph_destroy", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",
     "rsvdt_hip_bgram", "rsvdt_hip_bapply", "rsvdt_hip_bchol_inv", "rsvdt_hip_bdrop_mask", "rsvdt_hip_btrmm_upper",
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",

@@ -21,8 +21,9 @@
 // neighbours a row gathers sit in that XCD's L2.
 #include "common.h"
 
-// LDS budget of the fused small-operator stepper (y and two stage vectors)
-#define RSVDT_FUSED_LDS (96 * 1024)
+// LDS budget of the fused small-operator stepper (CSR matrix, y and two
+// stage vectors of one column)
+#define RSVDT_FUSED_LDS (160 * 1024)
 
 namespace rsvdt {
 
@@ -637,16 +638,32 @@ extern "C" int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage, const vo
 }
 
 // ---------------------------------------------------------------- fused small-operator stepping
-// For operators whose state fits in LDS (N x cw per workgroup), whole runs
-// of Horner steps in one launch: each workgroup owns cw columns of one block
-// (columns never interact in A*Y) and keeps y and the two stage vectors in
-// LDS, so a stage costs one barrier instead of one kernel launch.  The last
-// stage of step s adds g + s*gstride (the folded forcing sample; may be
-// null) and, when step i0+s stores a snapshot (SourceCode/SaveSnapshots.c:17-29),
-// writes y to snap[pos] in FP64.  Stage coefficients c[0..3] (h/4, h/3, h/2, h).
+// For operators that fit in LDS together with their state, whole runs of
+// Horner steps in one launch: each workgroup owns one column of one block
+// (columns never interact in A*Y), stages the CSR matrix and y and the two
+// stage vectors of its column in LDS, and a stage costs one barrier instead
+// of one kernel launch.  The last stage of step s adds g + s*gstride (the
+// folded forcing sample; may be null) and, when step i0+s stores a snapshot
+// (SourceCode/SaveSnapshots.c:17-29), writes y to snap[pos] in FP64.  Stage
+// coefficients c1..c4 = h/4, h/3, h/2, h.
+struct fused_lds {
+    size_t val, state, rp, ci, total;
+};
+__host__ __device__ inline size_t rnd16(size_t b) { return (b + 15) & ~(size_t)15; }
+__host__ __device__ inline fused_lds fused_layout(int64_t nrows, int64_t nnz, int a_real, int prec)
+{
+    fused_lds l;
+    l.val = 0;
+    l.state = rnd16((size_t)nnz * (a_real ? 8 : 16));
+    l.rp = l.state + rnd16((size_t)3 * (size_t)nrows * (prec == 64 ? 16 : 8));
+    l.ci = l.rp + rnd16((size_t)(nrows + 1) * 4);
+    l.total = l.ci + rnd16((size_t)nnz * 4);
+    return l;
+}
+
 template <typename T, bool AREAL>
 __global__ __launch_bounds__(256) void k_horner_fused(
-    int nrows, int k, int cw, int nsteps, int64_t bstride,
+    int nrows, int nnz, int k, int nsteps, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
     typename vec2<T>::type *y, const typename vec2<T>::type *__restrict__ g, int64_t gstride,
     double c1, double c2, double c3, double c4,
@@ -655,21 +672,20 @@ __global__ __launch_bounds__(256) void k_horner_fused(
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
     extern __shared__ double2 fsm[];
-    V *sy = (V *)fsm;
-    V *sa = sy + (int64_t)nrows * cw;
-    V *sb = sa + (int64_t)nrows * cw;
-    const int j0 = blockIdx.x * cw;
-    const int ncol = (k - j0) < cw ? (k - j0) : cw;
+    const fused_lds lay = fused_layout(nrows, nnz, AREAL ? 1 : 0, sizeof(T) == 8 ? 64 : 32);
+    char *base = (char *)fsm;
+    AV *sv = (AV *)(base + lay.val);
+    V *sy = (V *)(base + lay.state);
+    V *sa = sy + nrows;
+    V *sb = sa + nrows;
+    int *srp = (int *)(base + lay.rp);
+    int *sci = (int *)(base + lay.ci);
+    const int j = blockIdx.x;
     const int64_t boff = (int64_t)blockIdx.y * bstride;
     const AV *vals = (const AV *)vals_;
-    const int tot = nrows * cw;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int r = e / cw, c = e % cw;
-        V v;
-        v.x = 0; v.y = 0;
-        if (c < ncol) v = y[boff + (int64_t)r * k + j0 + c];
-        sy[e] = v;
-    }
+    for (int e = threadIdx.x; e < nnz; e += blockDim.x) { sv[e] = vals[e]; sci[e] = colidx[e]; }
+    for (int r = threadIdx.x; r <= nrows; r += blockDim.x) srp[r] = rowptr[r];
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) sy[r] = y[boff + (int64_t)r * k + j];
     __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
         const int64_t i = i0 + st;
@@ -684,80 +700,68 @@ __global__ __launch_bounds__(256) void k_horner_fused(
             const V *vin = stage == 0 ? sy : (stage == 2 ? sb : sa);
             V *out = stage == 3 ? sy : (stage == 1 ? sb : sa);
             const double cs = stage == 0 ? c1 : (stage == 1 ? c2 : (stage == 2 ? c3 : c4));
-            for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-                const int r = e / cw, c = e % cw;
+            for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
                 double sr = 0, si = 0;
-                const int e1 = rowptr[r + 1];
-                for (int p = rowptr[r]; p < e1; ++p) {
-                    const V x = vin[colidx[p] * cw + c];
+                const int e1 = srp[r + 1];
+                for (int p = srp[r]; p < e1; ++p) {
+                    const V x = vin[sci[p]];
                     const double xr = (double)x.x, xi = (double)x.y;
                     if constexpr (AREAL) {
-                        const double a = vals[p];
+                        const double a = sv[p];
                         sr += a * xr;
                         si += a * xi;
                     } else {
-                        const double2 a = vals[p];
+                        const double2 a = sv[p];
                         sr += a.x * xr - a.y * xi;
                         si += a.x * xi + a.y * xr;
                     }
                 }
-                const V yv = sy[e];
+                const V yv = sy[r];
                 double ox = (double)yv.x + cs * sr, oy = (double)yv.y + cs * si;
-                if (stage == 3 && gs != nullptr && c < ncol) {
-                    const V gv = gs[(int64_t)r * k + j0 + c];
+                if (stage == 3 && gs != nullptr) {
+                    const V gv = gs[(int64_t)r * k + j];
                     ox += (double)gv.x;
                     oy += (double)gv.y;
                 }
                 V o;
                 o.x = (T)ox;
                 o.y = (T)oy;
-                out[e] = o;       // stage 3 writes sy in place: each entry is read and written by one thread
-                if (stage == 3 && pos >= 0 && c < ncol)
-                    snap[pos * snap_stride + (int64_t)r * k + j0 + c] = make_double2((double)o.x, (double)o.y);
+                out[r] = o;       // stage 3 writes sy in place: each entry is read and written by one thread
+                if (stage == 3 && pos >= 0)
+                    snap[pos * snap_stride + (int64_t)r * k + j] = make_double2((double)o.x, (double)o.y);
             }
             __syncthreads();
         }
     }
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int r = e / cw, c = e % cw;
-        if (c < ncol) y[boff + (int64_t)r * k + j0 + c] = sy[e];
-    }
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) y[boff + (int64_t)r * k + j] = sy[r];
 }
 
-static int fused_cw(int64_t nrows, int64_t k, int prec)
+extern "C" int rsvdt_hip_horner_fused_fits(int64_t nrows, int64_t nnz, int a_real, int prec)
 {
-    const int64_t esz = prec == 64 ? 16 : 8;
-    int cw = 1;
-    while (cw * 2 <= k && cw < 8 && 3 * nrows * esz * (cw * 2) <= RSVDT_FUSED_LDS) cw *= 2;
-    return 3 * nrows * esz * cw <= RSVDT_FUSED_LDS ? cw : 0;
+    if (nrows <= 0 || nnz < 0 || nrows > 0x7fffffff || nnz > 0x7fffffff) return 0;
+    return fused_layout(nrows, nnz, a_real, prec).total <= (size_t)RSVDT_FUSED_LDS;
 }
 
-extern "C" int64_t rsvdt_hip_horner_fused_max_rows(int prec)
-{
-    return RSVDT_FUSED_LDS / (3 * (prec == 64 ? 16 : 8));
-}
-
-extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, void *y, const void *g, int64_t gstride,
+extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, void *y, const void *g, int64_t gstride,
                                       const double *coef, int64_t nsteps, double *snap, int64_t snap_stride,
                                       int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s)
 {
-    if (!d || d->nrows <= 0 || d->k <= 0 || d->nb <= 0 || d->nb > 65535 || !y || !coef ||
+    if (!d || d->nrows <= 0 || d->k <= 0 || d->k > 65535 || d->nb <= 0 || d->nb > 65535 || !y || !coef ||
         (snap && (RR <= 0 || Nstore <= 0)) || (d->prec != 64 && d->prec != 32)) {
         set_error_msg("rsvdt_hip_horner_fused: bad arguments");
         return RSVDT_HIP_ERR_ARG;
     }
-    const int cw = fused_cw(d->nrows, d->k, d->prec);
-    if (!cw) {
-        set_error_msg("rsvdt_hip_horner_fused: the operator is too large for the LDS-resident stepper");
+    if (!rsvdt_hip_horner_fused_fits(d->nrows, nnz, d->a_real, d->prec)) {
+        set_error_msg("rsvdt_hip_horner_fused: the operator and its state do not fit in LDS");
         return RSVDT_HIP_ERR_ARG;
     }
     if (nsteps <= 0) return RSVDT_HIP_OK;
-    const dim3 grid((unsigned)((d->k + cw - 1) / cw), (unsigned)d->nb);
-    const size_t lds = (size_t)3 * (size_t)d->nrows * (size_t)cw * (d->prec == 64 ? 16 : 8);
+    const dim3 grid((unsigned)d->k, (unsigned)d->nb);
+    const size_t lds = fused_layout(d->nrows, nnz, d->a_real, d->prec).total;
     const int64_t bs = d->nrows * d->k;
     hipStream_t st = as_stream(s);
 #define RSVDT_FUSED_LAUNCH(T, AR)                                                                                 \
-    hipLaunchKernelGGL((k_horner_fused<T, AR>), grid, dim3(256), lds, st, (int)d->nrows, (int)d->k, cw,            \
+    hipLaunchKernelGGL((k_horner_fused<T, AR>), grid, dim3(256), lds, st, (int)d->nrows, (int)nnz, (int)d->k,      \
                        (int)nsteps, bs, d->rowptr, d->col, d->val, (typename vec2<T>::type *)y,                    \
                        (const typename vec2<T>::type *)g, gstride, coef[0], coef[1], coef[2], coef[3],             \
                        (double2 *)snap, snap_stride, i0, Nt, RR, Nstore, dir)

@@ -394,7 +394,7 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     stepper st;
     RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
     /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
-    const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && N <= rsvdt_hip_horner_fused_max_rows(prec);
+    const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, prec);
     if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
         RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2));
         desc_for(&st.desc, A, dir, prec, k, 1);
@@ -434,7 +434,7 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         if (t_rk4) pt_begin(t_rk4);
         if (fused) {
             const double cf[4] = {st.h / 4, st.h / 3, st.h / 2, st.h};
-            int hr = rsvdt_hip_horner_fused(&st.desc, st.y, fslot(&st, buf, 0), (int64_t)N * k, cf, steps,
+            int hr = rsvdt_hip_horner_fused(&st.desc, A->nnz, st.y, fslot(&st, buf, 0), (int64_t)N * k, cf, steps,
                                             Yall ? Yall->d : NULL, (int64_t)N * k, i0, V->TS.Nt, V->TS.ResRatio,
                                             Nstore, dir, g_rsvdt.stream);
             if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
@@ -510,7 +510,7 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
     double *u2 = rsvdt_scratch(SCR_U2, blk * (size_t)G);
     double *acc = form ? rsvdt_scratch(SCR_ACC, blk * (size_t)G) : u2;
     if (!u1 || !u2 || !acc) return rsvdt_err(RSVDT_ERR_MEM, "TSExpADeltaT", "cannot allocate work buffers");
-    const int fused = form == 0 && g_rsvdt.opt.fuse_small && N <= rsvdt_hip_horner_fused_max_rows(64);
+    const int fused = form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, 64);
     for (PetscInt g0 = 0; g0 < nb; g0 += G) {
         PetscInt g = G < nb - g0 ? G : nb - g0;
         rsvdt_spmm_desc d;
@@ -520,7 +520,8 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         rsvdt_stream_t q = g_rsvdt.stream;
         if (fused) {
             const double cf[4] = {h / 4, h / 3, h / 2, h};
-            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, y, NULL, 0, cf, nsteps, NULL, 0, 0, 0, 1, 1, dir, q));
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, A->nnz, y, NULL, 0, cf, nsteps, NULL, 0, 0, 0, 1, 1,
+                                                              dir, q));
             continue;
         }
         for (PetscInt it = 0; it < nsteps; it++) {

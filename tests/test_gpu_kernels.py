@@ -246,7 +246,7 @@ def test_horner_stage_xcd_remaps(L, pkg, k, lanes):
 
 @pytest.mark.parametrize("N,k,real,prec,nb,snaps", [(700, 10, False, 64, 1, True), (500, 16, True, 64, 1, True),
                                                     (333, 3, False, 64, 4, False), (700, 10, False, 32, 1, True),
-                                                    (2048, 2, True, 64, 1, True), (90, 40, False, 32, 2, False)])
+                                                    (1200, 2, True, 64, 1, True), (90, 40, False, 32, 2, False)])
 def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
     """Whole runs of Horner steps in one launch (LDS-resident state) against
     numpy: y' = y + hA(y + h/2 A(y + h/3 A(y + h/4 Ay))) + g_s, with the FP64
@@ -284,22 +284,24 @@ def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
     dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
     coef = np.array([h / 4, h / 3, h / 2, h])
     d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
-    L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), V, V, I64, V, I64, V, I64, I64, I64, I64, I64,
+    L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), I64, V, V, I64, V, I64, V, I64, I64, I64, I64, I64,
                                          ctypes.c_int, V]
-    L.rsvdt_hip_horner_fused_max_rows.restype = I64
-    assert N <= L.rsvdt_hip_horner_fused_max_rows(prec)
-    rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), dy.p, dg.p if dg else None, N * k, coef.ctypes.data_as(V), nsteps,
-                                  dsnap.p if snaps else None, N * k, i0, Nt, RR, Nstore, dr, None)
+    L.rsvdt_hip_horner_fused_fits.argtypes = [I64, I64, ctypes.c_int, ctypes.c_int]
+    assert L.rsvdt_hip_horner_fused_fits(N, A.nnz, int(real), prec) == 1
+    rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), A.nnz, dy.p, dg.p if dg else None, N * k, coef.ctypes.data_as(V),
+                                  nsteps, dsnap.p if snaps else None, N * k, i0, Nt, RR, Nstore, dr, None)
     assert rc == 0, L.rsvdt_hip_last_error()
     out = dy.get(cdt, y.shape)
     tol = 1e-12 if prec == 64 else 3e-5
     assert rel(out, y) < tol
     if snaps:
         assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol
-    # an operator too large for the LDS-resident state is refused
-    big = Desc(L.rsvdt_hip_horner_fused_max_rows(prec) + 1, k, 1, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
-    assert L.rsvdt_hip_horner_fused(ctypes.byref(big), dy.p, None, 0, coef.ctypes.data_as(V), 1, None, 0, 0, 0, 1, 1,
-                                    1, None) != 0
+    # an operator too large for LDS is refused before any launch
+    nbig = 4000
+    assert L.rsvdt_hip_horner_fused_fits(nbig, 10 * nbig, int(real), prec) == 0
+    big = Desc(nbig, k, 1, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
+    assert L.rsvdt_hip_horner_fused(ctypes.byref(big), 10 * nbig, dy.p, None, 0, coef.ctypes.data_as(V), 1, None, 0,
+                                    0, 0, 1, 1, 1, None) != 0
 
 
 def _point_operator(P, F, rng, cplx):
