@@ -88,7 +88,8 @@ typedef struct {
                              batches; 5: 12 + nontemporal (variants apply when k <= lanes;
                              4 and 5 only to the Horner stage with lanes >= 16); 6: the
                              Horner stage on the row-grouped operator below (k <= lanes,
-                             16..64 lanes) */
+                             16..64 lanes); 7: the FP32 Horner stage with two adjacent
+                             columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes) */
     /* row-grouped form of the same operator (variant 6): groups of `group`
      * consecutive rows (2, 3, 4, 5 or 8); group q's distinct columns are
      * gword[gptr[q] .. gptr[q+1]-1], each (column << 8 | mask of the group's
@@ -138,11 +139,13 @@ int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *vin, const v
  * snapshot (i > Nt, (i-Nt) % RR == 0, column (i-Nt)/RR-1 mod Nstore,
  * reversed when dir == 0), writes y in FP64 to snap + pos*snap_stride
  * (complex entries; snap may be NULL).  One launch, one workgroup per
- * (column, block). */
+ * (column, block).  maxrow: the longest row's nonzero count (-1 if unknown);
+ * rows of at most 4 nonzeros in operators of at most 1024 rows keep their
+ * nonzeros in registers. */
 int rsvdt_hip_horner_fused_fits(int64_t nrows, int64_t nnz, int a_real, int prec);
-int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, void *y, const void *g, int64_t gstride,
-                           const double *coef, int64_t nsteps, double *snap, int64_t snap_stride, int64_t i0,
-                           int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s);
+int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, int64_t maxrow, void *y, const void *g,
+                           int64_t gstride, const double *coef, int64_t nsteps, double *snap, int64_t snap_stride,
+                           int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s);
 int rsvdt_hip_spmm_blocks(int64_t nrows, int64_t k, int64_t nb, const int32_t *rowptr, const int32_t *col,
                           const double *val_complex, const double *in, int64_t in_bstride,
                           double *out, int64_t out_bstride, rsvdt_stream_t s);

@@ -566,6 +566,124 @@ static int dispatch_horner_group(const rsvdt_spmm_desc *d, const void *vin, cons
     return RSVDT_HIP_ERR_ARG;
 }
 
+// ---------------------------------------------------------------- FP32 column pairs
+// The FP32 Horner stage with each lane owning two adjacent columns, so that
+// every gather is one 16-byte load (a float4: two complex FP32 values)
+// instead of two 8-byte ones: half the load instructions per gathered byte.
+// LPR = k/2 lanes per row (k even, k/2 a power of two), 8-nonzero batches,
+// nontemporal row-local streams.
+template <bool AREAL, int LPR, int NB>
+__device__ __forceinline__ void gather_batch_pair(int eb, int e1, int lane, int k, const int32_t *__restrict__ colidx,
+                                                  const typename aval<float, AREAL>::type *__restrict__ vals,
+                                                  const float4 *__restrict__ ub4, double (&sr)[2], double (&si)[2])
+{
+    using AV = typename aval<float, AREAL>::type;
+    constexpr int EPL = LPR >= NB ? 1 : NB / LPR;
+    int mc[EPL];
+    AV mv[EPL];
+#pragma unroll
+    for (int u = 0; u < EPL; ++u) {
+        const int e = eb + lane + u * LPR;
+        const bool ok = (LPR <= NB || lane < NB) && e < e1;
+        mc[u] = ok ? colidx[e] : 0;
+        if (ok) mv[u] = vals[e];
+        else mv[u] = AV{};
+    }
+    const int cnt = (e1 - eb) < NB ? (e1 - eb) : NB;
+    int cs[NB];
+    AV as[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        cs[u] = __shfl(mc[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+        as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
+    }
+    const int k2 = k >> 1;
+    float4 xs[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) xs[u] = ub4[(int64_t)cs[u] * k2 + lane];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        if (u < cnt) {
+            const double x0r = xs[u].x, x0i = xs[u].y, x1r = xs[u].z, x1i = xs[u].w;
+            if constexpr (AREAL) {
+                sr[0] += as[u] * x0r; si[0] += as[u] * x0i;
+                sr[1] += as[u] * x1r; si[1] += as[u] * x1i;
+            } else {
+                sr[0] += as[u].x * x0r - as[u].y * x0i; si[0] += as[u].x * x0i + as[u].y * x0r;
+                sr[1] += as[u].x * x1r - as[u].y * x1i; si[1] += as[u].x * x1i + as[u].y * x1r;
+            }
+        }
+    }
+}
+
+template <bool AREAL, int LPR>
+__global__ __launch_bounds__(256) void k_horner_pair(
+    int nrows, int k, int xcd_remap, int64_t bstride, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const void *__restrict__ vals_, const float2 *__restrict__ vin,
+    const float2 *yv, const double2 *__restrict__ ys, const float2 *__restrict__ g, float2 *out, double c,
+    double2 *__restrict__ snap, const int *__restrict__ snap_slot, int64_t snap_stride)
+{
+    using AV = typename aval<float, AREAL>::type;
+    constexpr int RPB = 256 / LPR, NB = 8;
+    const int bx = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
+    const int lane = threadIdx.x & (LPR - 1);
+    const int row = bx * RPB + (int)(threadIdx.x / LPR);
+    if (row >= nrows) return;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const AV *vals = (const AV *)vals_;
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+    if (snap_slot != nullptr) {
+        const int sl = *snap_slot;
+        snap = sl >= 0 ? snap + (int64_t)sl * snap_stride : nullptr;
+    }
+    const int64_t idx = boff + (int64_t)row * k + 2 * lane;     // first of the lane's two columns
+    const bool nty = (const void *)yv != (const void *)vin;
+    typedef float ev4f __attribute__((ext_vector_type(4)));
+    ev4f yv4 = nty ? __builtin_nontemporal_load((const ev4f *)(yv + idx)) : *(const ev4f *)(yv + idx);
+    ev4f gv4 = {0.f, 0.f, 0.f, 0.f};
+    if (g != nullptr) gv4 = __builtin_nontemporal_load((const ev4f *)(g + idx));
+    const float4 yr = make_float4(yv4.x, yv4.y, yv4.z, yv4.w), gr = make_float4(gv4.x, gv4.y, gv4.z, gv4.w);
+    double sr[2] = {0, 0}, si[2] = {0, 0};
+    const float4 *ub4 = (const float4 *)(vin + boff);
+    int eb = e0;
+    for (; e1 - eb >= NB; eb += NB) gather_batch_pair<AREAL, LPR, NB>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    const int rem = e1 - eb;
+    if (rem > NB / 2) gather_batch_pair<AREAL, LPR, NB>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    else if (rem > NB / 4) gather_batch_pair<AREAL, LPR, NB / 2>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    else if (rem > 0) gather_batch_pair<AREAL, LPR, NB / 4>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    double y0r = yr.x, y0i = yr.y, y1r = yr.z, y1i = yr.w;
+    if (ys != nullptr) {
+        const double a = ys[blockIdx.y].x, b = ys[blockIdx.y].y;
+        double t = a * y0r - b * y0i; y0i = a * y0i + b * y0r; y0r = t;
+        t = a * y1r - b * y1i; y1i = a * y1i + b * y1r; y1r = t;
+    }
+    float4 o;
+    o.x = (float)(y0r + c * sr[0] + (double)gr.x);
+    o.y = (float)(y0i + c * si[0] + (double)gr.y);
+    o.z = (float)(y1r + c * sr[1] + (double)gr.z);
+    o.w = (float)(y1i + c * si[1] + (double)gr.w);
+    *(float4 *)(out + idx) = o;
+    if (snap != nullptr) {
+        st_stream(snap + idx, make_double2((double)o.x, (double)o.y), true);
+        st_stream(snap + idx + 1, make_double2((double)o.z, (double)o.w), true);
+    }
+}
+
+template <bool AREAL, int L>
+static int launch_horner_pair(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
+                              double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
+{
+    constexpr int RPB = 256 / L;
+    const unsigned gx = ceil_div(d->nrows, RPB);
+    const unsigned gy = (unsigned)d->nb;
+    const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
+    hipLaunchKernelGGL((k_horner_pair<AREAL, L>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows, (int)d->k, remap,
+                       d->nrows * d->k, d->rowptr, d->col, d->val, (const float2 *)vin, (const float2 *)y,
+                       (const double2 *)ys, (const float2 *)g, (float2 *)out, c, (double2 *)snap, slot, sstride);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
+    return RSVDT_HIP_OK;
+}
+
 template <typename T, bool AREAL>
 static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
                            double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
@@ -576,6 +694,19 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
             return RSVDT_HIP_ERR_ARG;
         }
         return dispatch_horner_group<T, AREAL>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+    }
+    if (d->variant == 7) {
+        if constexpr (sizeof(T) == 4) {
+            const int64_t l = d->k / 2;
+            if (d->k % 2 == 0) {
+                if (l == 4) return launch_horner_pair<AREAL, 4>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+                if (l == 8) return launch_horner_pair<AREAL, 8>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+                if (l == 16) return launch_horner_pair<AREAL, 16>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+                if (l == 32) return launch_horner_pair<AREAL, 32>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+            }
+        }
+        set_error_msg("rsvdt_hip_horner_stage: variant 7 needs the FP32 state and k/2 a power of two in 4..32");
+        return RSVDT_HIP_ERR_ARG;
     }
     int lpr = d->lanes;
     const int k = (int)d->k;
@@ -736,13 +867,105 @@ __global__ __launch_bounds__(256) void k_horner_fused(
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) y[boff + (int64_t)r * k + j] = sy[r];
 }
 
+// The same with each thread's rows (at most RPT, 512 threads) and their
+// nonzeros (at most MAXNZ per row) held in registers for the whole launch:
+// a stage is then MAXNZ LDS reads and FMAs per row and one barrier.
+template <typename T, bool AREAL, int MAXNZ, int RPT>
+__global__ __launch_bounds__(512) void k_horner_fused_reg(
+    int nrows, int k, int nsteps, int64_t bstride,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
+    typename vec2<T>::type *y, const typename vec2<T>::type *__restrict__ g, int64_t gstride,
+    double c1, double c2, double c3, double c4,
+    double2 *__restrict__ snap, int64_t snap_stride, int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir)
+{
+    using V = typename vec2<T>::type;
+    using AV = typename aval<T, AREAL>::type;
+    extern __shared__ double2 fsm[];
+    V *sy = (V *)fsm;
+    V *sa = sy + nrows;
+    V *sb = sa + nrows;
+    const int j = blockIdx.x;
+    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    const AV *vals = (const AV *)vals_;
+    int cl[RPT][MAXNZ];
+    AV av[RPT][MAXNZ];
+    bool live[RPT];
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+        const int r = threadIdx.x + t * blockDim.x;
+        live[t] = r < nrows;
+        const int e0 = live[t] ? rowptr[r] : 0, n = live[t] ? rowptr[r + 1] - e0 : 0;
+#pragma unroll
+        for (int u = 0; u < MAXNZ; ++u) {
+            cl[t][u] = u < n ? colidx[e0 + u] : 0;      // padding: column 0 with a zero value
+            if (u < n) av[t][u] = vals[e0 + u];
+            else av[t][u] = AV{};
+        }
+        if (live[t]) sy[r] = y[boff + (int64_t)r * k + j];
+    }
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+        const int64_t i = i0 + st;
+        int64_t pos = -1;
+        if (snap != nullptr && i > Nt && (i - Nt) % RR == 0) {
+            pos = ((i - Nt) / RR - 1) % Nstore;
+            if (!dir) pos = Nstore - 1 - pos;
+        }
+        const V *gs = g != nullptr ? g + (int64_t)st * gstride + boff : nullptr;
+#pragma unroll 1
+        for (int stage = 0; stage < 4; ++stage) {
+            const V *vin = stage == 0 ? sy : (stage == 2 ? sb : sa);
+            V *out = stage == 3 ? sy : (stage == 1 ? sb : sa);
+            const double cs = stage == 0 ? c1 : (stage == 1 ? c2 : (stage == 2 ? c3 : c4));
+#pragma unroll
+            for (int t = 0; t < RPT; ++t) {
+                if (!live[t]) continue;
+                const int r = threadIdx.x + t * blockDim.x;
+                double sr = 0, si = 0;
+#pragma unroll
+                for (int u = 0; u < MAXNZ; ++u) {
+                    const V x = vin[cl[t][u]];
+                    const double xr = (double)x.x, xi = (double)x.y;
+                    if constexpr (AREAL) {
+                        sr += av[t][u] * xr;
+                        si += av[t][u] * xi;
+                    } else {
+                        sr += av[t][u].x * xr - av[t][u].y * xi;
+                        si += av[t][u].x * xi + av[t][u].y * xr;
+                    }
+                }
+                const V yv = sy[r];
+                double ox = (double)yv.x + cs * sr, oy = (double)yv.y + cs * si;
+                if (stage == 3 && gs != nullptr) {
+                    const V gv = gs[(int64_t)r * k + j];
+                    ox += (double)gv.x;
+                    oy += (double)gv.y;
+                }
+                V o;
+                o.x = (T)ox;
+                o.y = (T)oy;
+                out[r] = o;
+                if (stage == 3 && pos >= 0)
+                    snap[pos * snap_stride + (int64_t)r * k + j] = make_double2((double)o.x, (double)o.y);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+        const int r = threadIdx.x + t * blockDim.x;
+        if (live[t]) y[boff + (int64_t)r * k + j] = sy[r];
+    }
+}
+
 extern "C" int rsvdt_hip_horner_fused_fits(int64_t nrows, int64_t nnz, int a_real, int prec)
 {
     if (nrows <= 0 || nnz < 0 || nrows > 0x7fffffff || nnz > 0x7fffffff) return 0;
     return fused_layout(nrows, nnz, a_real, prec).total <= (size_t)RSVDT_FUSED_LDS;
 }
 
-extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, void *y, const void *g, int64_t gstride,
+extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, int64_t maxrow, void *y, const void *g,
+                                      int64_t gstride,
                                       const double *coef, int64_t nsteps, double *snap, int64_t snap_stride,
                                       int64_t i0, int64_t Nt, int64_t RR, int64_t Nstore, int dir, rsvdt_stream_t s)
 {
@@ -757,6 +980,25 @@ extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, voi
     }
     if (nsteps <= 0) return RSVDT_HIP_OK;
     const dim3 grid((unsigned)d->k, (unsigned)d->nb);
+    hipStream_t st0 = as_stream(s);
+    if (maxrow >= 0 && maxrow <= 4 && d->nrows <= 1024) {    // rows and nonzeros in registers
+        const size_t lds3 = (size_t)3 * (size_t)d->nrows * (d->prec == 64 ? 16 : 8);
+#define RSVDT_FUSED_REG(T, AR)                                                                                    \
+        hipLaunchKernelGGL((k_horner_fused_reg<T, AR, 4, 2>), grid, dim3(512), lds3, st0, (int)d->nrows, (int)d->k,  \
+                           (int)nsteps, d->nrows * d->k, d->rowptr, d->col, d->val, (typename vec2<T>::type *)y,    \
+                           (const typename vec2<T>::type *)g, gstride, coef[0], coef[1], coef[2], coef[3],          \
+                           (double2 *)snap, snap_stride, i0, Nt, RR, Nstore, dir)
+        if (d->prec == 64) {
+            if (d->a_real) RSVDT_FUSED_REG(double, true);
+            else RSVDT_FUSED_REG(double, false);
+        } else {
+            if (d->a_real) RSVDT_FUSED_REG(float, true);
+            else RSVDT_FUSED_REG(float, false);
+        }
+#undef RSVDT_FUSED_REG
+        RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_fused");
+        return RSVDT_HIP_OK;
+    }
     const size_t lds = fused_layout(d->nrows, nnz, d->a_real, d->prec).total;
     const int64_t bs = d->nrows * d->k;
     hipStream_t st = as_stream(s);

@@ -90,6 +90,9 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     if (g_rsvdt.opt.spmm_lanes) d->lanes = g_rsvdt.opt.spmm_lanes;
     if (g_rsvdt.opt.spmm_variant >= 0) d->variant = g_rsvdt.opt.spmm_variant;
     if (d->variant == 6 && (!d->group || g_rsvdt.opt.rk4_form)) d->variant = 3;   /* no grouped form */
+    if (d->variant == 7 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 2 || !(k / 2 == 4 || k / 2 == 8 ||
+                                                                         k / 2 == 16 || k / 2 == 32)))
+        d->variant = 3;                                                         /* no column pairs */
 }
 
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
@@ -363,6 +366,18 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
                 if (ms < best) { best = ms; bl = l; bv = v; br = r; }
             }
     }
+    if (!f && prec == 32 && k % 2 == 0 && (k / 2 == 4 || k / 2 == 8 || k / 2 == 16 || k / 2 == 32)) {
+        for (int r = 0; r < 4; r++) {              /* FP32 column pairs (lanes = k/2) */
+            dsc.lanes = (int)(k / 2); dsc.variant = 7; dsc.xcd_remap = r;
+            if (rsvdt_hip_horner_stage(&dsc, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream)) continue;
+            rsvdt_hip_event_record(e0, g_rsvdt.stream);
+            for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&dsc, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream);
+            rsvdt_hip_event_record(e1, g_rsvdt.stream);
+            float ms = 0;
+            rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+            if (ms < best) { best = ms; bl = (int)(k / 2); bv = 7; br = r; }
+        }
+    }
     rsvdt_hip_event_destroy(e0);
     rsvdt_hip_event_destroy(e1);
     A->tuned_lanes[f][p][q] = bl;
@@ -434,7 +449,8 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
         if (t_rk4) pt_begin(t_rk4);
         if (fused) {
             const double cf[4] = {st.h / 4, st.h / 3, st.h / 2, st.h};
-            int hr = rsvdt_hip_horner_fused(&st.desc, A->nnz, st.y, fslot(&st, buf, 0), (int64_t)N * k, cf, steps,
+            int hr = rsvdt_hip_horner_fused(&st.desc, A->nnz, dir ? A->maxrow : A->maxrowH, st.y, fslot(&st, buf, 0),
+                                            (int64_t)N * k, cf, steps,
                                             Yall ? Yall->d : NULL, (int64_t)N * k, i0, V->TS.Nt, V->TS.ResRatio,
                                             Nstore, dir, g_rsvdt.stream);
             if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
@@ -520,8 +536,8 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         rsvdt_stream_t q = g_rsvdt.stream;
         if (fused) {
             const double cf[4] = {h / 4, h / 3, h / 2, h};
-            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, A->nnz, y, NULL, 0, cf, nsteps, NULL, 0, 0, 0, 1, 1,
-                                                              dir, q));
+            RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, A->nnz, dir ? A->maxrow : A->maxrowH, y, NULL, 0, cf,
+                                                              nsteps, NULL, 0, 0, 0, 1, 1, dir, q));
             continue;
         }
         for (PetscInt it = 0; it < nsteps; it++) {

@@ -39,6 +39,7 @@ struct _p_RSVDT_Mat {
     /* row-grouped form of A and A^H for the grouped-gather stage kernel
      * (group = 0: not built, the grouping would not save gathers) */
     int group, groupH;
+    PetscInt maxrow, maxrowH;   /* longest row of A and of A^H (nonzeros) */
     int32_t *d_gptr, *d_gptrH;
     uint32_t *d_gword, *d_gwordH;
     void *d_gval, *d_gvalH;

@@ -8,18 +8,69 @@
 #include <string.h>
 #include <unistd.h>
 
-static PetscErrorCode view_submat(Mat M, PetscInt c0, PetscInt nc, const char *path)
+static void put_be64o(unsigned char *p, uint64_t v)
 {
-    /* columns c0 .. c0+nc-1 of a dense Mat, through the host */
-    PetscInt rows = M->rows;
-    double *all = malloc(sizeof(double) * 2 * (size_t)(rows * M->cols));
-    if (!all) return rsvdt_err(RSVDT_ERR_MEM, "SaveResults", "out of memory");
-    PetscErrorCode ierr = RSVDT_MatGetDense(M, all);
-    Mat S = NULL;
-    if (!ierr) ierr = RSVDT_MatCreateDense(rows, nc, all + 2 * (size_t)(c0 * rows), &S);
-    free(all);
-    if (!ierr) ierr = RSVDT_MatView(S, path);
-    RSVDT_MatDestroy(&S);
+    for (int i = 7; i >= 0; i--) { p[i] = (unsigned char)(v & 0xff); v >>= 8; }
+}
+
+/* MatView of columns c0 .. c0+nc-1 of a dense Mat (PETSc's dense binary
+ * layout: the AIJ one with every entry stored), streamed: the columns are
+ * gathered on the device into one rows x nc row-major buffer, which goes to
+ * the host and the file in slices of rows, so a result file costs its own
+ * size in transfers and a slice of host memory. */
+static PetscErrorCode view_cols(Mat M, PetscInt c0, PetscInt nc, const char *path)
+{
+    const PetscInt rows = M->rows, kb = M->kb, nb = M->cols / M->kb;
+    if (nc <= 0 || c0 < 0 || c0 + nc > M->cols) return rsvdt_err(RSVDT_ERR_USER, "SaveResults", "bad column range");
+    double *T = NULL;
+    if (rsvdt_hip_malloc((void **)&T, (size_t)(rows * nc > 0 ? rows * nc : 1) * 16))
+        return rsvdt_err(RSVDT_ERR_MEM, "SaveResults", "cannot allocate the column buffer: %s", rsvdt_hip_last_error());
+    PetscErrorCode ierr = 0;
+    for (PetscInt ci = 0; ci < nc && !ierr; ci++) {
+        const PetscInt c = c0 + ci;
+        const PetscInt b = M->order ? c % nb : c / kb, j = M->order ? c / nb : c % kb;
+        int h = rsvdt_hip_zcopy3(1, rows, 1, M->d + 2 * ((size_t)b * (size_t)rows * (size_t)kb + (size_t)j), 0, kb, 0,
+                                 T + 2 * (size_t)ci, 0, nc, 0, g_rsvdt.stream);
+        if (h) ierr = rsvdt_err(h, "SaveResults", "%s", rsvdt_hip_last_error());
+    }
+    if (!ierr && rsvdt_hip_sync()) ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "SaveResults", "%s", rsvdt_hip_last_error());
+    FILE *f = ierr ? NULL : fopen(path, "wb");
+    if (!ierr && !f) ierr = rsvdt_err(RSVDT_ERR_FILE, "SaveResults", "cannot create '%s'", path);
+    const PetscInt slice = (PetscInt)((64u << 20) / (16 * (size_t)nc)) + 1;   /* rows per slice: ~64 MB of values */
+    const size_t bufb = (size_t)(slice < rows ? slice : rows) * (size_t)nc * 16 + 64;
+    unsigned char *buf = ierr ? NULL : malloc(bufb);
+    double *hv = ierr ? NULL : malloc(bufb);
+    if (!ierr && (!buf || !hv)) ierr = rsvdt_err(RSVDT_ERR_MEM, "SaveResults", "out of memory");
+    if (!ierr) {
+        int64_t hdr[4] = {1211216, rows, nc, rows * nc};
+        for (int i = 0; i < 4; i++) put_be64o(buf + 8 * i, (uint64_t)hdr[i]);
+        fwrite(buf, 8, 4, f);
+        for (PetscInt r0 = 0; r0 < rows; r0 += slice) {            /* row lengths */
+            const PetscInt nr = slice < rows - r0 ? slice : rows - r0;
+            for (PetscInt r = 0; r < nr; r++) put_be64o(buf + 8 * r, (uint64_t)nc);
+            fwrite(buf, 8, (size_t)nr, f);
+        }
+        for (PetscInt c = 0; c < nc; c++) put_be64o(buf + 8 * c, (uint64_t)c);   /* column indices, row by row */
+        for (PetscInt r = 0; r < rows; r++) fwrite(buf, 8, (size_t)nc, f);
+        for (PetscInt r0 = 0; r0 < rows && !ierr; r0 += slice) {   /* values, row-major */
+            const PetscInt nr = slice < rows - r0 ? slice : rows - r0;
+            const size_t n = (size_t)nr * (size_t)nc * 2;
+            if (rsvdt_hip_memcpy_d2h(hv, T + 2 * (size_t)r0 * (size_t)nc, n * 8)) {
+                ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "SaveResults", "%s", rsvdt_hip_last_error());
+                break;
+            }
+            for (size_t i = 0; i < n; i++) {
+                uint64_t u;
+                memcpy(&u, &hv[i], 8);
+                put_be64o(buf + 8 * i, u);
+            }
+            fwrite(buf, 8, n, f);
+        }
+    }
+    if (f && fclose(f) != 0 && !ierr) ierr = rsvdt_err(RSVDT_ERR_FILE, "SaveResults", "write error on '%s'", path);
+    free(buf);
+    free(hv);
+    rsvdt_hip_free(T);
     return ierr;
 }
 
@@ -42,18 +93,18 @@ PetscErrorCode SaveResults(Resolvent_matrices *Res, RSVDt_vars *V, Directories *
         RSVDT_CHK(PermuteMat(Res->V_hat, V));
         for (PetscInt ik = 0; ik < k; ik++) {
             snprintf(dirs->IO_dir, PETSC_MAX_PATH_LEN, "%s%s%d%s", dirs->FolderDir, "U_hat_k", (int)ik + 1, "_allFreqs");
-            RSVDT_CHK(view_submat(Res->U_hat, ik * Nwe, Nwe, dirs->IO_dir));
+            RSVDT_CHK(view_cols(Res->U_hat, ik * Nwe, Nwe, dirs->IO_dir));
             snprintf(dirs->IO_dir, PETSC_MAX_PATH_LEN, "%s%s%d%s", dirs->FolderDir, "V_hat_k", (int)ik + 1, "_allFreqs");
-            RSVDT_CHK(view_submat(Res->V_hat, ik * Nwe, Nwe, dirs->IO_dir));
+            RSVDT_CHK(view_cols(Res->V_hat, ik * Nwe, Nwe, dirs->IO_dir));
         }
     } else {
         rsvdt_printf("%d matrices of size %d x %d for response modes\n%d matrices of size %d x %d for forcing modes\n\n",
                      (int)Nwe, (int)Res->U_hat->rows, (int)k, (int)Nwe, (int)Res->V_hat->rows, (int)k);
         for (PetscInt iw = 0; iw < Nwe; iw++) {
             snprintf(dirs->IO_dir, PETSC_MAX_PATH_LEN, "%s%s%d%s", dirs->FolderDir, "U_hat_Freq", (int)iw, "_allK");
-            RSVDT_CHK(view_submat(Res->U_hat, iw * k, k, dirs->IO_dir));
+            RSVDT_CHK(view_cols(Res->U_hat, iw * k, k, dirs->IO_dir));
             snprintf(dirs->IO_dir, PETSC_MAX_PATH_LEN, "%s%s%d%s", dirs->FolderDir, "V_hat_Freq", (int)iw, "_allK");
-            RSVDT_CHK(view_submat(Res->V_hat, iw * k, k, dirs->IO_dir));
+            RSVDT_CHK(view_cols(Res->V_hat, iw * k, k, dirs->IO_dir));
         }
     }
     RSVDT_CHK(SaveInputVarsCopy(dirs));

@@ -608,6 +608,9 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
                          &m->d_val64));
     RSVDT_CHK(upload_grouped(m->rows, m->cols, m->h_rowptr, m->h_col, m->h_val, m->is_real, &m->group, &m->d_gptr,
                              &m->d_gword, &m->d_gval));
+    m->maxrow = 0;
+    for (PetscInt r = 0; r < m->rows; r++)
+        if (m->h_rowptr[r + 1] - m->h_rowptr[r] > m->maxrow) m->maxrow = m->h_rowptr[r + 1] - m->h_rowptr[r];
     PetscInt nr = m->cols, nnz = m->nnz;
     int64_t *rp = calloc((size_t)nr + 1, sizeof(int64_t));
     int64_t *ci = malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
@@ -628,6 +631,9 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     PetscErrorCode ierr = upload_csr(nr, rp, ci, v, nnz, m->is_real, &m->d_rowptrH, &m->d_colH, &m->d_valH64);
     if (!ierr) ierr = upload_grouped(nr, m->rows, rp, ci, v, m->is_real, &m->groupH, &m->d_gptrH, &m->d_gwordH,
                                      &m->d_gvalH);
+    m->maxrowH = 0;
+    for (PetscInt r = 0; r < nr; r++)
+        if (rp[r + 1] - rp[r] > m->maxrowH) m->maxrowH = rp[r + 1] - rp[r];
     free(rp); free(ci); free(v); free(fill);
     memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
     return ierr;

@@ -47,8 +47,11 @@ def test_full_sketch_exact_gains(pkg):
     assert np.allclose(S, kat.exact_transfer_sigmas(A, Nw, w, dt), rtol=1e-10)
 
 
-def test_cli_drop_in(pkg, tmp_path):
-    """rsvdt -inputs variables.yaml with InputForcingFlg, PETSc-binary in/out."""
+@pytest.mark.parametrize("opt", [2, 1])
+def test_cli_drop_in(pkg, tmp_path, opt):
+    """rsvdt -inputs variables.yaml with InputForcingFlg, PETSc-binary in/out;
+    results per frequency (SaveResultsOpt 2: U_hat_Freq<m>_allK, 0-based) or
+    per mode (1: U_hat_k<j>_allFreqs, 1-based), SourceCode/SaveResults.c:32-92."""
     N, k, Nw = 60, 3, 8
     A = kat.gl_operator(N=N, L=20.0)
     rng = np.random.default_rng(4)
@@ -68,7 +71,7 @@ TransientLength: 2
 dt: 0.02
 TransientRemoval: true
 Display: 1
-SaveResultsOpt: 2
+SaveResultsOpt: {opt}
 InputForcingFlg: true
 InputForcingDir: F_hat
 TransICDir: none
@@ -85,9 +88,23 @@ TransICDir: none
     Uo, Vo, So = o.run(F)
     assert S.shape == (k, Nw)
     assert np.allclose(S, So, rtol=1e-10)
-    U0 = petsc_bin.read_dense(str(out / "U_hat_Freq0_allK"))
-    assert U0.shape == (N, k)
-    assert abs(abs(np.vdot(U0[:, 0], Uo[:, 0])) - 1) < 1e-9
+    Nwe = o.Nw_eff
+    if opt == 2:
+        for m in (0, Nwe - 1):
+            Um = petsc_bin.read_dense(str(out / f"U_hat_Freq{m}_allK"))
+            Vm = petsc_bin.read_dense(str(out / f"V_hat_Freq{m}_allK"))
+            assert Um.shape == (N, k) and Vm.shape == (N, k)
+            for j in range(k):
+                assert abs(abs(np.vdot(Um[:, j], Uo[:, m * k + j])) - 1) < 1e-9
+                assert abs(abs(np.vdot(Vm[:, j], Vo[:, m * k + j])) - 1) < 1e-9
+    else:
+        for j in range(k):
+            Uj = petsc_bin.read_dense(str(out / f"U_hat_k{j + 1}_allFreqs"))
+            Vj = petsc_bin.read_dense(str(out / f"V_hat_k{j + 1}_allFreqs"))
+            assert Uj.shape == (N, Nwe) and Vj.shape == (N, Nwe)
+            for m in range(Nwe):
+                assert abs(abs(np.vdot(Uj[:, m], Uo[:, m * k + j])) - 1) < 1e-9
+                assert abs(abs(np.vdot(Vj[:, m], Vo[:, m * k + j])) - 1) < 1e-9
     assert (out / "vars.yaml").read_text() == yaml
 
 

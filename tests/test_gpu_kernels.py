@@ -182,7 +182,9 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
     (10, False, 64, 3, 0, 0, True, True), (16, False, 64, 2, 8, 0, False, False), (32, True, 64, 1, 0, 2, False, True),
     (40, False, 64, 1, 0, 0, True, False), (16, True, 32, 1, 0, 3, False, True), (16, True, 32, 1, 8, 0, False, True),
     (10, False, 32, 2, 4, 0, True, True), (16, True, 64, 1, 16, 4, False, True), (16, False, 64, 2, 16, 5, True, True),
-    (32, True, 32, 1, 32, 4, False, True), (12, True, 64, 1, 16, 5, False, False)])
+    (32, True, 32, 1, 32, 4, False, True), (12, True, 64, 1, 16, 5, False, False),
+    (16, True, 32, 1, 8, 7, False, True), (32, False, 32, 2, 16, 7, True, True), (8, True, 32, 3, 4, 7, True, False),
+    (64, False, 32, 1, 32, 7, False, True)])
 def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced):
     """out_b = ys_b y_b + c A vin_b + g_b, with the snapshot copy, against numpy."""
     print(f"horner k={k} real={real} prec={prec} nb={nb} lanes={lanes} variant={variant} scaled={scaled} "
@@ -244,18 +246,24 @@ def test_horner_stage_xcd_remaps(L, pkg, k, lanes):
         assert np.array_equal(o, outs[0])
 
 
-@pytest.mark.parametrize("N,k,real,prec,nb,snaps", [(700, 10, False, 64, 1, True), (500, 16, True, 64, 1, True),
-                                                    (333, 3, False, 64, 4, False), (700, 10, False, 32, 1, True),
-                                                    (1200, 2, True, 64, 1, True), (90, 40, False, 32, 2, False)])
-def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
+@pytest.mark.parametrize("N,k,real,prec,nb,snaps,band", [
+    (700, 10, False, 64, 1, True, False), (500, 16, True, 64, 1, True, False), (333, 3, False, 64, 4, False, False),
+    (700, 10, False, 32, 1, True, False), (1200, 2, True, 64, 1, True, False), (90, 40, False, 32, 2, False, False),
+    (500, 10, False, 64, 1, True, True), (1024, 4, True, 32, 2, False, True), (1000, 3, False, 64, 1, True, True)])
+def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps, band):
     """Whole runs of Horner steps in one launch (LDS-resident state) against
     numpy: y' = y + hA(y + h/2 A(y + h/3 A(y + h/4 Ay))) + g_s, with the FP64
-    snapshots of the steps that store one."""
-    print(f"fused N={N} k={k} real={real} prec={prec} nb={nb} snaps={snaps}", flush=True)
+    snapshots of the steps that store one; band: a periodic tridiagonal
+    operator (at most 3 nonzeros per row, the register-resident variant)."""
+    print(f"fused N={N} k={k} real={real} prec={prec} nb={nb} snaps={snaps} band={band}", flush=True)
     rng = np.random.default_rng(N + k)
-    A = sp.random(N, N, density=4.0 / N, random_state=k, format="csr", dtype=np.float64)
-    if not real:
-        A = A + 1j * sp.random(N, N, density=2.0 / N, random_state=k + 1, format="csr")
+    if band:
+        d = [rng.standard_normal(N) + (0 if real else 1j * rng.standard_normal(N)) for _ in range(3)]
+        A = sp.diags([d[0][1:], d[1] - 4, d[2][:-1], d[0][:1], d[2][-1:]], [-1, 0, 1, N - 1, -(N - 1)])
+    else:
+        A = sp.random(N, N, density=4.0 / N, random_state=k, format="csr", dtype=np.float64)
+        if not real:
+            A = A + 1j * sp.random(N, N, density=2.0 / N, random_state=k + 1, format="csr")
     A = (A - 2 * sp.eye(N)).tocsr().astype(np.complex128)
     A.sort_indices()
     h, nsteps = 0.05, 23
@@ -284,24 +292,30 @@ def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps):
     dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
     coef = np.array([h / 4, h / 3, h / 2, h])
     d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
-    L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), I64, V, V, I64, V, I64, V, I64, I64, I64, I64, I64,
-                                         ctypes.c_int, V]
+    L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), I64, I64, V, V, I64, V, I64, V, I64, I64, I64, I64,
+                                         I64, ctypes.c_int, V]
     L.rsvdt_hip_horner_fused_fits.argtypes = [I64, I64, ctypes.c_int, ctypes.c_int]
     assert L.rsvdt_hip_horner_fused_fits(N, A.nnz, int(real), prec) == 1
-    rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), A.nnz, dy.p, dg.p if dg else None, N * k, coef.ctypes.data_as(V),
-                                  nsteps, dsnap.p if snaps else None, N * k, i0, Nt, RR, Nstore, dr, None)
-    assert rc == 0, L.rsvdt_hip_last_error()
-    out = dy.get(cdt, y.shape)
+    maxrow = int(np.max(np.diff(A.indptr)))
     tol = 1e-12 if prec == 64 else 3e-5
-    assert rel(out, y) < tol
-    if snaps:
-        assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol
+    # maxrow -1: nonzeros staged in LDS; the true maxrow: in registers when the rows are short enough
+    for mr in (-1, maxrow):
+        dy = Dev(pkg, y0.astype(cdt))
+        dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
+        rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), A.nnz, mr, dy.p, dg.p if dg else None, N * k,
+                                      coef.ctypes.data_as(V), nsteps, dsnap.p if snaps else None, N * k, i0, Nt, RR,
+                                      Nstore, dr, None)
+        assert rc == 0, L.rsvdt_hip_last_error()
+        out = dy.get(cdt, y.shape)
+        assert rel(out, y) < tol, mr
+        if snaps:
+            assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol
     # an operator too large for LDS is refused before any launch
     nbig = 4000
     assert L.rsvdt_hip_horner_fused_fits(nbig, 10 * nbig, int(real), prec) == 0
     big = Desc(nbig, k, 1, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
-    assert L.rsvdt_hip_horner_fused(ctypes.byref(big), 10 * nbig, dy.p, None, 0, coef.ctypes.data_as(V), 1, None, 0,
-                                    0, 0, 1, 1, 1, None) != 0
+    assert L.rsvdt_hip_horner_fused(ctypes.byref(big), 10 * nbig, 10, dy.p, None, 0, coef.ctypes.data_as(V), 1, None,
+                                    0, 0, 0, 1, 1, 1, None) != 0
 
 
 def _point_operator(P, F, rng, cplx):

@@ -40,7 +40,7 @@ def operator(name):
 
 
 @pytest.mark.parametrize("name", ["flow2d", "flow3d"])
-@pytest.mark.parametrize("prec,variant,remap", [(64, 3, 1), (64, 0, 0), (32, 2, 1), (64, 6, 0), (32, 6, 3)])
+@pytest.mark.parametrize("prec,variant,remap", [(64, 3, 1), (64, 0, 0), (32, 2, 1), (64, 6, 0), (32, 6, 3), (32, 7, 0)])
 def test_horner_stage_full_size(L, pkg, name, prec, variant, remap):
     print(f"horner full size {name} prec={prec} variant={variant} remap={remap}", flush=True)
     A, k = operator(name)
