@@ -294,8 +294,8 @@ static int dispatch_lanes(const rsvdt_spmm_desc *d, int stage, const void *uin, 
 // One stage: out_b = ys_b * y_b + c * (A vin_b) (+ g_b), ys_b a complex scale
 // per block (1 when ys is null; the forcing precompute uses it), optional
 // FP64 copy of out into snap.
-template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
-__global__ __launch_bounds__(256) void k_horner_stage(
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT, int BS = 256>
+__global__ __launch_bounds__(BS) void k_horner_stage(
     int nrows, int k, int xcd_remap, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
     const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void k_horner_stage(
 {
     using V = typename vec2<T>::type;
     using AV = typename aval<T, AREAL>::type;
-    constexpr int RPB = 256 / LPR;
+    constexpr int RPB = BS / LPR;
     const int bx = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
     const int lane = threadIdx.x & (LPR - 1);
     const int row = bx * RPB + (int)(threadIdx.x / LPR);
@@ -359,16 +359,16 @@ __global__ __launch_bounds__(256) void k_horner_stage(
     }
 }
 
-template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT>
+template <typename T, bool AREAL, int LPR, int CPL, int NB, bool NT, int BS = 256>
 static int launch_horner(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
                          double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
     using V = typename vec2<T>::type;
-    constexpr int RPB = 256 / LPR;
+    constexpr int RPB = BS / LPR;
     const unsigned gx = ceil_div(d->nrows, RPB);
     const unsigned gy = (unsigned)d->nb;
     const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
-    hipLaunchKernelGGL((k_horner_stage<T, AREAL, LPR, CPL, NB, NT>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows,
+    hipLaunchKernelGGL((k_horner_stage<T, AREAL, LPR, CPL, NB, NT, BS>), dim3(gx, gy), dim3(BS), 0, s, (int)d->nrows,
                        (int)d->k, remap, d->nrows * d->k, d->rowptr, d->col, d->val, (const V *)vin, (const V *)y,
                        (const double2 *)ys, (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
@@ -721,6 +721,8 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
         if constexpr (C == 1 && L >= 16) {                                                                        \
             if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);     \
             if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);      \
+            if (var == 8) return launch_horner<T, AREAL, L, C, 8, true, 512>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);  \
+            if (var == 9) return launch_horner<T, AREAL, L, C, 8, true, 1024>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
         }                                                                                                         \
         if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
         if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \

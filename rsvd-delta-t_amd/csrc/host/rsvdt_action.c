@@ -348,7 +348,7 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         if ((k + l - 1) / l > 4) continue;            /* at most 4 columns per lane */
         if (l >= 2 * k && l > 4) continue;            /* no more than half the lanes idle */
         const int cpl = (int)((k + l - 1) / l);
-        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? (dsc.group ? 7 : 6) : 4);
+        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? 10 : 4);
         for (int v = 0; v < nvar; v++)
             for (int r = 0; r < 4; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
