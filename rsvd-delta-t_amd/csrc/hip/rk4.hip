@@ -721,8 +721,6 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
         if constexpr (C == 1 && L >= 16) {                                                                        \
             if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);     \
             if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);      \
-            if (var == 8) return launch_horner<T, AREAL, L, C, 8, true, 512>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);  \
-            if (var == 9) return launch_horner<T, AREAL, L, C, 8, true, 1024>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
         }                                                                                                         \
         if (C == 1 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
         if (C == 1 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \

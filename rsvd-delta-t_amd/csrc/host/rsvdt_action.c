@@ -97,7 +97,8 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
 
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
  * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
- * while the two chunk buffers would take more than a fifth of free memory. */
+ * while the two chunk buffers would take more than 40 % of free memory (they
+ * are freed when the stepping ends, before the transient removal's work). */
 static PetscInt chunk_samples(int form, PetscInt S) { return form ? 2 * S + 1 : S; }
 
 static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
@@ -107,7 +108,7 @@ static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
     rsvdt_hip_mem_info(&fr, &tot);
     double blk = (double)N * (double)k * (double)esz;
     PetscInt S = form ? 31 : 64;
-    while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.2 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
+    while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.4 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
     return S;
 }
 
@@ -348,7 +349,7 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         if ((k + l - 1) / l > 4) continue;            /* at most 4 columns per lane */
         if (l >= 2 * k && l > 4) continue;            /* no more than half the lanes idle */
         const int cpl = (int)((k + l - 1) / l);
-        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? 10 : 4);
+        const int nvar = cpl > 1 ? 1 : ((!f && l >= 16) ? 8 : 4);
         for (int v = 0; v < nvar; v++)
             for (int r = 0; r < 4; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
