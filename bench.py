@@ -89,6 +89,38 @@ def calib_copy(pkg, gib=1, reps=5):
                                      "launches": reps, "kernel": "k_zcopy3"}}))
 
 
+def stream_ceiling(pkg, gib=1, reps=10):
+    """The HBM rate a plain streaming kernel with the stage kernel's mix
+    reaches on this GPU: y += a x over two gib-GiB complex arrays
+    (rsvdt_hip_zaxpy: two reads and one write per element), timed with HIP
+    events; GB/s of bytes moved."""
+    import ctypes
+    L = pkg.lib()
+    V = ctypes.c_void_p
+    L.rsvdt_hip_event_create.argtypes = [ctypes.POINTER(V)]
+    L.rsvdt_hip_event_record.argtypes = [V, V]
+    L.rsvdt_hip_event_elapsed_ms.argtypes = [V, V, ctypes.POINTER(ctypes.c_float)]
+    L.rsvdt_hip_zaxpy.argtypes = [ctypes.c_int64, ctypes.c_double, V, V, V]
+    n = (gib << 30) // 16
+    x, y, e0, e1 = V(), V(), V(), V()
+    for p in (x, y):
+        if L.rsvdt_hip_malloc(ctypes.byref(p), ctypes.c_size_t(n * 16)) != 0:
+            raise RuntimeError("device allocation failed")
+        L.rsvdt_hip_zero(p, ctypes.c_size_t(n * 16), None)
+    L.rsvdt_hip_event_create(ctypes.byref(e0)); L.rsvdt_hip_event_create(ctypes.byref(e1))
+    L.rsvdt_hip_zaxpy(n, 0.5, x, y, None)
+    L.rsvdt_hip_event_record(e0, None)
+    for _ in range(reps):
+        L.rsvdt_hip_zaxpy(n, 0.5, x, y, None)
+    L.rsvdt_hip_event_record(e1, None)
+    L.rsvdt_hip_sync()
+    ms = ctypes.c_float()
+    L.rsvdt_hip_event_elapsed_ms(e0, e1, ctypes.byref(ms))
+    L.rsvdt_hip_event_destroy(e0); L.rsvdt_hip_event_destroy(e1)
+    L.rsvdt_hip_free(x); L.rsvdt_hip_free(y)
+    return round(3 * 16 * n * reps / (ms.value * 1e-3) / 1e9, 1)
+
+
 def time_to_modes_gl(pkg, nmembers):
     """Time to sigma_1 relative error <= 1e-6 on held-out members of the
     Ginzburg-Landau family (N=500, k=16, Nw=64, transient removal): the plain
@@ -246,6 +278,9 @@ def main():
         out["time_to_modes_s"] = {"fp64": round(res[64][0], 3), "fp32_state": round(res[32][0], 3),
                                   "speedup": round(res[64][0] / res[32][0], 3), "sigma1_max_rel_err_fp32": err}
     s.close()
+    sc = stream_ceiling(pkg)
+    out["roofline"]["stream_2r1w_GBps"] = sc
+    out["roofline"]["frac_of_stream"] = round(achieved / sc, 4)
     if not args.no_ttm and args.workload == "flow2d":
         out["time_to_modes"] = time_to_modes_gl(pkg, args.ttm_members)
     sys.stdout.flush()

@@ -1055,27 +1055,36 @@ static PetscErrorCode apply_sparse(Mat M, int herm, Mat *Y, PetscInt k)
     PetscInt nb = (*Y)->cols / k;
     Mat out;
     RSVDT_CHK(rsvdt_mat_alloc_dense(rows, (*Y)->cols, k, 0, &out));
-    /* complex FP64 values: the device copies hold real values when the matrix is real */
-    const void *val = herm ? M->d_valH64 : M->d_val64;
-    double *vz = (double *)val;
-    int own = 0;
+    /* the block SpMM takes complex FP64 values; the device copies of a real
+     * matrix hold real values, so its complex expansion is built once (from
+     * the host copy) and kept on the Mat */
+    const double *vz = herm ? M->d_valH64 : M->d_val64;
     if (M->is_real) {
-        /* expand the real values to complex for the block SpMM */
-        size_t nnz = (size_t)M->nnz;
-        double *h = malloc(sizeof(double) * 2 * (nnz ? nnz : 1));
-        double *hr = malloc(sizeof(double) * (nnz ? nnz : 1));
-        if (!h || !hr) return rsvdt_err(RSVDT_ERR_MEM, "ApplyWeightMats", "out of memory");
-        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_memcpy_d2h(hr, val, sizeof(double) * nnz));
-        for (size_t e = 0; e < nnz; e++) { h[2 * e] = hr[e]; h[2 * e + 1] = 0; }
-        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_malloc((void **)&vz, sizeof(double) * 2 * (nnz ? nnz : 1)));
-        RSVDT_HIPCHK("ApplyWeightMats", rsvdt_hip_memcpy_h2d(vz, h, sizeof(double) * 2 * nnz));
-        free(h); free(hr);
-        own = 1;
+        double **pz = herm ? &M->d_valzH : &M->d_valz;
+        if (!*pz) {
+            size_t nnz = (size_t)M->nnz;
+            double *h = malloc(sizeof(double) * 2 * (nnz ? nnz : 1));
+            if (!h) return rsvdt_err(RSVDT_ERR_MEM, "ApplyWeightMats", "out of memory");
+            if (!herm) {
+                for (size_t e = 0; e < nnz; e++) { h[2 * e] = M->h_val[2 * e]; h[2 * e + 1] = 0; }
+            } else {       /* A^H of a real matrix: its transpose, in the order of the uploaded A^H */
+                double *hr = malloc(sizeof(double) * (nnz ? nnz : 1));
+                if (!hr) { free(h); return rsvdt_err(RSVDT_ERR_MEM, "ApplyWeightMats", "out of memory"); }
+                int rc = rsvdt_hip_memcpy_d2h(hr, M->d_valH64, sizeof(double) * nnz);
+                if (rc) { free(h); free(hr); return rsvdt_err(rc, "ApplyWeightMats", "%s", rsvdt_hip_last_error()); }
+                for (size_t e = 0; e < nnz; e++) { h[2 * e] = hr[e]; h[2 * e + 1] = 0; }
+                free(hr);
+            }
+            int rc = rsvdt_hip_malloc((void **)pz, sizeof(double) * 2 * (nnz ? nnz : 1));
+            if (!rc) rc = rsvdt_hip_memcpy_h2d(*pz, h, sizeof(double) * 2 * nnz);
+            free(h);
+            if (rc) return rsvdt_err(rc, "ApplyWeightMats", "%s", rsvdt_hip_last_error());
+        }
+        vz = *pz;
     }
     RSVDT_HIPCHK("ApplyWeightMats",
                  rsvdt_hip_spmm_blocks(rows, k, nb, herm ? M->d_rowptrH : M->d_rowptr, herm ? M->d_colH : M->d_col, vz,
                                        (*Y)->d, cols * k, out->d, rows * k, g_rsvdt.stream));
-    if (own) { rsvdt_hip_sync(); rsvdt_hip_free(vz); }
     RSVDT_MatDestroy(Y);
     *Y = out;
     return 0;

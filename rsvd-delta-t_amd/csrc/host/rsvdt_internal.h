@@ -36,6 +36,7 @@ struct _p_RSVDT_Mat {
     double *h_val;           /* 2*nnz */
     int32_t *d_rowptr, *d_col, *d_rowptrH, *d_colH;
     void *d_val64, *d_valH64;
+    double *d_valz, *d_valzH;   /* complex expansion of real values (weight products), built on first use */
     /* row-grouped form of A and A^H for the grouped-gather stage kernel
      * (group = 0: not built, the grouping would not save gathers) */
     int group, groupH;

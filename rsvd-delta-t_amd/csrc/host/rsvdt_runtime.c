@@ -430,6 +430,7 @@ PetscErrorCode RSVDT_MatDestroy(Mat *pm)
     rsvdt_hip_free(m->d_rowptrH); rsvdt_hip_free(m->d_colH);
     rsvdt_hip_free(m->d_val64);
     rsvdt_hip_free(m->d_valH64);
+    rsvdt_hip_free(m->d_valz); rsvdt_hip_free(m->d_valzH);
     rsvdt_hip_free(m->d_gptr); rsvdt_hip_free(m->d_gword); rsvdt_hip_free(m->d_gval);
     rsvdt_hip_free(m->d_gptrH); rsvdt_hip_free(m->d_gwordH); rsvdt_hip_free(m->d_gvalH);
     free(m->h_rowptr); free(m->h_col); free(m->h_val);
