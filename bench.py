@@ -270,8 +270,12 @@ def main():
             s2 = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, precision=prec)
             s2.random_sketch(seed=1373, gaussian=True)
             t0 = time.time()
-            s2.direct(); s2.power_iteration(); s2.svd_before_adjoint(); s2.store_u(); s2.adjoint(); s2.svd_allfreqs()
-            pkg.lib().rsvdt_hip_sync()
+            for name, fn in (("direct", s2.direct), ("power iteration", s2.power_iteration),
+                             ("svd", s2.svd_before_adjoint), ("store U", s2.store_u), ("adjoint", s2.adjoint),
+                             ("svd all", s2.svd_allfreqs)):
+                fn()
+                pkg.lib().rsvdt_hip_sync()
+                print(f"[bench] FP{prec} {name} done at {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
             res[prec] = (time.time() - t0, pkg.mat_to_dense(pkg._vp(s2.Res.S_hat)).real)
             s2.close()
         err = float(np.max(np.abs(res[32][1][0] - res[64][1][0]) / res[64][1][0]))
