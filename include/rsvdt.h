@@ -241,6 +241,7 @@ typedef struct {
                              precomputed term per step); 1: the reference's stage order */
     int fuse_small;       /* operators whose state fits in LDS: whole chunks of steps per launch */
     int spmm_variant;     /* -1: autotuned; else force the stage-kernel variant (rsvdt_hip.h) */
+    int tr_group;         /* most test vectors per transient-removal group (0: sized to free memory) */
 } RSVDT_Options;
 PetscErrorCode RSVDT_GetOptions(RSVDT_Options *o);
 PetscErrorCode RSVDT_SetOptions(const RSVDT_Options *o);

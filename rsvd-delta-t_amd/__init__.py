@@ -91,7 +91,7 @@ class Directories(ctypes.Structure):
 class RSVDT_Options(ctypes.Structure):
     _fields_ = [("precision", _int), ("chunk_steps", _int), ("spmm_lanes", _int), ("xcd_remap", _int),
                 ("use_graph", _int), ("autotune", _int), ("verbose", _int), ("overlap_forcing", _int), ("rk4_form", _int),
-                ("fuse_small", _int), ("spmm_variant", _int)]
+                ("fuse_small", _int), ("spmm_variant", _int), ("tr_group", _int)]
 
 
 class RSVDT_ActionStats(ctypes.Structure):
@@ -314,7 +314,7 @@ def get_options():
 
 # engine option defaults (RSVDT_Initialize's); a Session starts from these
 OPTION_DEFAULTS = dict(precision=64, chunk_steps=0, spmm_lanes=0, xcd_remap=1, use_graph=0, autotune=1, verbose=0,
-                       overlap_forcing=1, rk4_form=0, fuse_small=1, spmm_variant=-1)
+                       overlap_forcing=1, rk4_form=0, fuse_small=1, spmm_variant=-1, tr_group=0)
 
 
 def set_options(**kw):

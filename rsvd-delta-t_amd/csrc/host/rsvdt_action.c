@@ -89,7 +89,7 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     }
     if (g_rsvdt.opt.spmm_lanes) d->lanes = g_rsvdt.opt.spmm_lanes;
     if (g_rsvdt.opt.spmm_variant >= 0) d->variant = g_rsvdt.opt.spmm_variant;
-    if (d->variant == 6 && (!d->group || g_rsvdt.opt.rk4_form)) d->variant = 3;   /* no grouped form */
+    if (d->variant == 6 && (!d->group || g_rsvdt.opt.rk4_form || k > 64)) d->variant = 3;   /* no grouped form */
     if (d->variant == 7 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 2 || !(k / 2 == 4 || k / 2 == 8 ||
                                                                          k / 2 == 16 || k / 2 == 32)))
         d->variant = 3;                                                         /* no column pairs */
@@ -97,8 +97,9 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
 
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
  * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
- * while the two chunk buffers would take more than 40 % of free memory (they
- * are freed when the stepping ends, before the transient removal's work). */
+ * while the two chunk buffers would take more than 60 % of free memory (an
+ * action frees large ones when its stepping ends, before the transient
+ * removal's work). */
 static PetscInt chunk_samples(int form, PetscInt S) { return form ? 2 * S + 1 : S; }
 
 static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
@@ -106,9 +107,11 @@ static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
     if (g_rsvdt.opt.chunk_steps > 0) return g_rsvdt.opt.chunk_steps;
     size_t fr = 0, tot = 0;
     rsvdt_hip_mem_info(&fr, &tot);
+    /* the chunk buffers of an earlier action are reused (or replaced) */
+    fr += rsvdt_scratch_size(SCR_FBUF) + rsvdt_scratch_size(SCR_FBUF2);
     double blk = (double)N * (double)k * (double)esz;
     PetscInt S = form ? 31 : 64;
-    while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.4 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
+    while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.6 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
     return S;
 }
 
@@ -497,6 +500,12 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_sync());
     for (int b = 0; b < 2; b++) rsvdt_hip_graph_destroy(gexec[b]);
     stepper_free(&st);
+    /* in an action, chunk buffers of more than 1 GiB go back to the device:
+     * the transient removal that follows sizes its groups to the free memory */
+    if (Yall && rsvdt_scratch_size(SCR_FBUF) + rsvdt_scratch_size(SCR_FBUF2) > ((size_t)1 << 30)) {
+        rsvdt_scratch_release(SCR_FBUF);
+        rsvdt_scratch_release(SCR_FBUF2);
+    }
     RSVDT_CHK(ierr);
     g_rsvdt.stats.steps = rend;
     g_rsvdt.stats.rk4_launches = launches;
@@ -733,28 +742,67 @@ PetscErrorCode DFT(Mat Yall, DFT_matrices *D, RSVD_matrices *R, RSVDt_vars *V)
  * the same QR in both, SourceCode/CreateMtilde.c:43-46 and
  * GalerkinProjAllModes.c:31-33); otherwise each recomputes it per group. */
 typedef struct {
-    const void *key;             /* Y_all data the cached group was built from */
+    const void *key;             /* Y_all data the group in X, V was built from (NULL: none) */
     PetscInt b0, kb, N, n;
-    double *X, *V;               /* kb x N x n each */
+    size_t cap;                  /* bytes of each buffer */
+    double *X, *V, *E, *T;       /* kb x N x n each; E and T are free between builds */
 } tr_group;
 static tr_group g_trg = {0};
 
+/* phase times of the transient removal (ms), kept only with the Verbose
+ * option, which synchronizes the device around each phase */
+enum { TRT_BASIS = 0, TRT_STEPS, TRT_MTILDE, TRT_GALERKIN, TRT_N };
+static double g_trt[TRT_N];
+static PetscInt g_trt_kb;
+static double trt_mark(void)
+{
+    if (!g_rsvdt.opt.verbose) return 0;
+    rsvdt_hip_sync();
+    return RSVDT_Time();
+}
+static void trt_add(int phase, double t0)
+{
+    if (!g_rsvdt.opt.verbose) return;
+    rsvdt_hip_sync();
+    g_trt[phase] += 1e3 * (RSVDT_Time() - t0);
+}
+
 static void tr_group_free(tr_group *g)
 {
-    if (g->X || g->V) rsvdt_hip_sync();
+    if (g->X || g->V || g->E || g->T) rsvdt_hip_sync();
     if (g->X) rsvdt_hip_free(g->X);
     if (g->V) rsvdt_hip_free(g->V);
+    if (g->E) rsvdt_hip_free(g->E);
+    if (g->T) rsvdt_hip_free(g->T);
     memset(g, 0, sizeof(*g));
 }
 
-/* test vectors per group: X, V, two basis work arrays, M V and two stepping
- * temporaries per test vector must fit in half of the free memory */
+/* The four group buffers are allocated once and reused by every group of
+ * CreateMtilde and GalerkinProjAllModes (allocating and freeing buffers of
+ * several GB per group costs more than the group's basis). */
+static PetscErrorCode tr_group_reserve(tr_group *g, size_t bytes)
+{
+    if (g->cap >= bytes && g->X && g->V && g->E && g->T) return 0;
+    tr_group_free(g);
+    if (rsvdt_hip_malloc((void **)&g->X, bytes) || rsvdt_hip_malloc((void **)&g->V, bytes) ||
+        rsvdt_hip_malloc((void **)&g->E, bytes) || rsvdt_hip_malloc((void **)&g->T, bytes)) {
+        tr_group_free(g);
+        return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate the snapshot basis (%.2f GB each)", bytes / 1e9);
+    }
+    g->cap = bytes;
+    return 0;
+}
+
+/* test vectors per group: the four group buffers, the stepping temporaries
+ * and the Galerkin output rows of a test vector must fit in half of the
+ * free memory (buffers already held by the group count as free) */
 static PetscInt tr_group_size(PetscInt k, PetscInt N, PetscInt n)
 {
     size_t fr = 0, tot = 0;
     rsvdt_hip_mem_info(&fr, &tot);
-    const double per = 7.0 * (double)N * (double)n * 16.0;
-    PetscInt kb = (PetscInt)(0.5 * (double)fr / per);
+    const double per = 6.0 * (double)N * (double)n * 16.0;
+    PetscInt kb = (PetscInt)(0.5 * ((double)fr + 4.0 * (double)g_trg.cap) / per);
+    if (g_rsvdt.opt.tr_group > 0 && kb > g_rsvdt.opt.tr_group) kb = g_rsvdt.opt.tr_group;
     if (kb < 1) kb = 1;
     if (kb > k) kb = k;
     const PetscInt ng = (k + kb - 1) / kb;
@@ -765,19 +813,14 @@ static PetscInt tr_group_size(PetscInt k, PetscInt N, PetscInt n)
 static PetscErrorCode tr_group_build(Mat Yall, PetscInt k, PetscInt b0, PetscInt kb, tr_group *g)
 {
     const PetscInt N = Yall->rows, n = Yall->cols / k;
-    if (g->key == (const void *)Yall->d && g->b0 == b0 && g->kb == kb && g->X && g->V) return 0;
-    tr_group_free(g);
+    if (g->key == (const void *)Yall->d && g->b0 == b0 && g->kb == kb && g->N == N && g->n == n) return 0;
+    g->key = NULL;
     if (Yall->order != 1 || Yall->kb != k) RSVDT_CHK(rsvdt_mat_to_layout(Yall, k, 1));
     const size_t bytes = (size_t)kb * (size_t)N * (size_t)n * 16;
-    double *E = NULL, *T = NULL;
-    if (rsvdt_hip_malloc((void **)&g->X, bytes) || rsvdt_hip_malloc((void **)&g->V, bytes) ||
-        rsvdt_hip_malloc((void **)&E, bytes) || rsvdt_hip_malloc((void **)&T, bytes)) {
-        if (E) rsvdt_hip_free(E);
-        tr_group_free(g);
-        return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate the snapshot basis (%.2f GB each)", bytes / 1e9);
-    }
+    RSVDT_CHK(tr_group_reserve(g, bytes));
     g->b0 = b0; g->kb = kb; g->N = N; g->n = n;
     PetscErrorCode ierr = 0;
+    double t0 = trt_mark();
     int h = rsvdt_hip_transpose_rows(N, n, k, b0, kb, Yall->d, g->X, 0, g_rsvdt.stream);
     if (h) ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
     /* The snapshots of one period are a periodic response of rank <= Nw_eff
@@ -785,54 +828,67 @@ static PetscErrorCode tr_group_build(Mat Yall, PetscInt k, PetscInt b0, PetscInt
      * (singular values down to rounding level); the basis spans it to
      * rounding accuracy with the numerically dependent directions left out
      * (zero columns). */
-    if (!ierr) ierr = rsvdt_span_basis(kb, N, n, g->X, g->V, E, T, N * n, n, 1, bytes, 1e-14);
-    rsvdt_hip_sync();
-    rsvdt_hip_free(E);
-    rsvdt_hip_free(T);
-    if (ierr) { tr_group_free(g); return ierr; }
+    if (!ierr) ierr = rsvdt_span_basis(kb, N, n, g->X, g->V, g->E, g->T, N * n, n, 1, bytes, 1e-14);
+    trt_add(TRT_BASIS, t0);
+    if (ierr) return ierr;
     g->key = Yall->d;
     return 0;
+}
+
+/* M_tilde of the test vectors b0 .. b0+g-1, whose basis the group holds
+ * (SourceCode/CreateMtilde.c:46-51). */
+static PetscErrorCode mtilde_group(LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR, PetscInt N, PetscInt Ns1,
+                                   PetscInt b0, PetscInt g)
+{
+    /* e^{A dT} V: the group's basis vectors as Ns1 blocks of N x g (the
+     * snapshot layout), stepped by the same block SpMM as the actions.
+     * Measured on a 2M-row operator this is faster than stepping each test
+     * vector's N x Ns1 basis as one wide block (4.8 s against 6.6 s per
+     * removal): a block of g columns stays in the last-level cache while its
+     * rows are gathered.  The basis work arrays E and T are free once the
+     * basis is built. */
+    double *Vblk = g_trg.E, *MV = g_trg.T;
+    double t0 = trt_mark();
+    int h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, g_trg.V, Vblk, 1, g_rsvdt.stream);
+    if (h) return rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
+    RSVDT_CHK(rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, Vblk, N, g, Ns1));
+    trt_add(TRT_STEPS, t0);
+    t0 = trt_mark();
+    h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, Vblk, MV, 0, g_rsvdt.stream);
+    if (h) return rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
+    if (!work) return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate work");
+    h = rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, N * Ns1, Ns1, 1, MV, N * Ns1, Ns1, 1,
+                        TSR->M_tilde_all->d + 2 * (size_t)b0 * (size_t)Ns1 * (size_t)Ns1, work, g_rsvdt.stream);
+    if (h) return rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
+    trt_add(TRT_MTILDE, t0);
+    return 0;
+}
+
+static PetscErrorCode mtilde_alloc(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR)
+{
+    PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1;
+    if (Yall->cols != Ns1 * k) return rsvdt_err(RSVDT_ERR_USER, "CreateMtilde", "Y_all must have (Nw+1)*k columns");
+    RSVDT_CHK(rsvdt_aij_upload(LNS->A));
+    RSVDT_MatDestroy(&TSR->M_tilde_all);
+    return rsvdt_mat_alloc_dense(Ns1, Ns1 * k, Ns1, 0, &TSR->M_tilde_all);
 }
 
 /* SourceCode/CreateMtilde.c:41-54: M_tilde_ik = V_ik^H e^{A dT} V_ik for every
  * test vector, e^{A dT} being RR homogeneous RK4 steps (TSExpADeltaT.c). */
 PetscErrorCode CreateMtilde(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR)
 {
+    RSVDT_CHK(mtilde_alloc(Yall, LNS, V, TSR));
     PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1, N = Yall->rows;
-    if (Yall->cols != Ns1 * k) return rsvdt_err(RSVDT_ERR_USER, "CreateMtilde", "Y_all must have (Nw+1)*k columns");
-    RSVDT_CHK(rsvdt_aij_upload(LNS->A));
-    RSVDT_MatDestroy(&TSR->M_tilde_all);
-    RSVDT_CHK(rsvdt_mat_alloc_dense(Ns1, Ns1 * k, Ns1, 0, &TSR->M_tilde_all));
     const PetscInt kb = tr_group_size(k, N, Ns1);
-    const size_t gbytes = (size_t)kb * (size_t)N * (size_t)Ns1 * 16;
-    double *MV = NULL, *Vblk = NULL;
-    if (rsvdt_hip_malloc((void **)&MV, gbytes) || rsvdt_hip_malloc((void **)&Vblk, gbytes)) {
-        if (MV) rsvdt_hip_free(MV);
-        return rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate M V (%.2f GB)", gbytes / 1e9);
-    }
+    g_trt_kb = kb;
     PetscErrorCode ierr = 0;
     for (PetscInt b0 = 0; b0 < k && !ierr; b0 += kb) {
         const PetscInt g = kb < k - b0 ? kb : k - b0;
         ierr = tr_group_build(Yall, k, b0, g, &g_trg);
-        if (ierr) break;
-        /* the group's basis vectors as Ns1 blocks of N x g (the snapshot
-         * layout), stepped by the same k-column SpMM as the actions */
-        int h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, g_trg.V, Vblk, 1, g_rsvdt.stream);
-        if (h) { ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error()); break; }
-        ierr = rsvdt_homogeneous_steps(LNS->A, V->TS.DirAdj ? 1 : 0, V->TS.dt, V->TS.ResRatio, Vblk, N, g, Ns1);
-        if (ierr) break;
-        h = rsvdt_hip_transpose_rows(N, Ns1, g, 0, g, Vblk, MV, 0, g_rsvdt.stream);
-        if (h) { ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error()); break; }
-        void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
-        if (!work) { ierr = rsvdt_err(RSVDT_ERR_MEM, "CreateMtilde", "cannot allocate work"); break; }
-        h = rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, N * Ns1, Ns1, 1, MV, N * Ns1, Ns1, 1,
-                            TSR->M_tilde_all->d + 2 * (size_t)b0 * (size_t)Ns1 * (size_t)Ns1, work, g_rsvdt.stream);
-        if (h) ierr = rsvdt_err(h, "CreateMtilde", "device call failed: %s", rsvdt_hip_last_error());
-        if (kb < k) tr_group_free(&g_trg);
+        if (!ierr) ierr = mtilde_group(LNS, V, TSR, N, Ns1, b0, g);
     }
     rsvdt_hip_sync();
-    rsvdt_hip_free(MV);
-    rsvdt_hip_free(Vblk);
     if (ierr) tr_group_free(&g_trg);
     return ierr;
 }
@@ -882,16 +938,33 @@ PetscErrorCode QRDecomposition(Mat Vm)
  *   swapped for the adjoint), ph = e^{+-i w m dT},
  *   a  = (ph I - M_tilde)^{-1} V^H (ph y1 - y2),   Y_hat_m = y1 - V a.
  * With the window coefficients C1 (y1 = X C1) and Ct (ph y1 - y2 = X Ct),
- * per test vector: B = (V^H X) Ct, a = solve, Y_hat = X C1 - V a. */
-PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR, DFT_matrices *D)
+ * per test vector: B = (V^H X) Ct, a = solve, Y_hat = X C1 - V a.
+ * The state below is shared by the groups of one projection. */
+typedef struct {
+    PetscInt k, N, Ns1, Nwe, kb;
+    double *dC, *dph, *Ybrm;      /* window coefficients (C1, Ct), shifts, one group's output rows */
+    double *B, *W;
+    int *info;
+    Mat Yup;
+} gal_ctx;
+
+static void gal_free(gal_ctx *c)
 {
-    (void)LNS;
+    rsvdt_hip_sync();
+    if (c->dC) rsvdt_hip_free(c->dC);
+    if (c->dph) rsvdt_hip_free(c->dph);
+    if (c->Ybrm) rsvdt_hip_free(c->Ybrm);
+    c->dC = c->dph = c->Ybrm = NULL;
+}
+
+static PetscErrorCode gal_begin(gal_ctx *c, Mat Yall, RSVDt_vars *V, DFT_matrices *D, PetscInt kb)
+{
+    memset(c, 0, sizeof(*c));
     PetscInt k = V->RSVD.k, Nw = V->RSVD.Nw, Ns1 = Nw + 1, Nwe = V->RSVD.Nw_eff, N = Yall->rows;
-    if (!TSR->M_tilde_all) return rsvdt_err(RSVDT_ERR_USER, "GalerkinProjAllModes", "M_tilde_all is not set");
+    c->k = k; c->N = N; c->Ns1 = Ns1; c->Nwe = Nwe; c->kb = kb;
     int dir = V->TS.DirAdj ? 1 : 0;
     double dT = V->TS.dt * (double)V->TS.ResRatio;
     double RR = (double)V->TS.ResRatio;
-    const PetscInt kb = tr_group_size(k, N, Ns1);
     /* host: dft (Nw x Nwe), the window coefficients for every test vector of a
      * group (C1 then Ct, kb x Ns1 x Nwe each) and the shifts ph (k x Nwe) */
     double *dft = malloc(sizeof(double) * 2 * (size_t)(Nw * Nwe));
@@ -922,65 +995,89 @@ PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_r
     free(dft);
     const size_t csz = sizeof(double) * 2 * (size_t)(2 * kb * Ns1 * Nwe);
     const size_t ybytes = (size_t)kb * (size_t)N * (size_t)Nwe * 16;
-    double *dC = NULL, *dph = NULL, *Ybrm = NULL;
     PetscErrorCode ierr = 0;
-    if (rsvdt_hip_malloc((void **)&dC, csz) || rsvdt_hip_malloc((void **)&dph, sizeof(double) * 2 * (size_t)(k * Nwe)) ||
-        rsvdt_hip_malloc((void **)&Ybrm, ybytes))
+    if (rsvdt_hip_malloc((void **)&c->dC, csz) || rsvdt_hip_malloc((void **)&c->dph, sizeof(double) * 2 * (size_t)(k * Nwe)) ||
+        rsvdt_hip_malloc((void **)&c->Ybrm, ybytes))
         ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProjAllModes", "cannot allocate work");
-    if (!ierr && (rsvdt_hip_memcpy_h2d(dC, C, csz) || rsvdt_hip_memcpy_h2d(dph, ph, sizeof(double) * 2 * (size_t)(k * Nwe))))
+    if (!ierr && (rsvdt_hip_memcpy_h2d(c->dC, C, csz) || rsvdt_hip_memcpy_h2d(c->dph, ph, sizeof(double) * 2 * (size_t)(k * Nwe))))
         ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProjAllModes", "device call failed: %s", rsvdt_hip_last_error());
     free(C); free(ph);
-    Mat Yup = NULL;
-    if (!ierr) ierr = rsvdt_mat_alloc_dense(N, Nwe * k, k, 1, &Yup);
+    if (!ierr) ierr = rsvdt_mat_alloc_dense(N, Nwe * k, k, 1, &c->Yup);
     size_t nb_b = (size_t)kb * (size_t)Ns1 * (size_t)Nwe * 16, nb_w = (size_t)kb * (size_t)Ns1 * (size_t)Ns1 * 16;
-    double *B = rsvdt_scratch(SCR_MISC, nb_b);
-    double *W = rsvdt_scratch(SCR_MISC2, nb_w);
-    int *info = rsvdt_scratch(SCR_GINFO, 256);
-    if (!ierr && (!B || !W || !info))
-        ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
+    c->B = rsvdt_scratch(SCR_MISC, nb_b);
+    c->W = rsvdt_scratch(SCR_MISC2, nb_w);
+    c->info = rsvdt_scratch(SCR_GINFO, 256);
+    if (!ierr && (!c->B || !c->W || !c->info)) ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
+    if (!ierr && rsvdt_hip_zero(c->info, 256, g_rsvdt.stream))
+        ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
+    if (ierr) { gal_free(c); RSVDT_MatDestroy(&c->Yup); }
+    return ierr;
+}
+
+/* The projection for test vectors b0 .. b0+g-1, whose snapshots and basis
+ * the group holds. */
+static PetscErrorCode gal_group(gal_ctx *c, TS_removal_matrices *TSR, PetscInt b0, PetscInt g)
+{
+    const PetscInt N = c->N, Ns1 = c->Ns1, Nwe = c->Nwe, kb = c->kb;
+    const int64_t xs = N * Ns1;
     rsvdt_stream_t q = g_rsvdt.stream;
-    if (!ierr && rsvdt_hip_zero(info, 256, q)) ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
-    const double *C1 = dC, *Ct = dC + 2 * (size_t)(kb * Ns1 * Nwe);
+    const double t0 = trt_mark();
+    /* work sizes follow the group's own batch count */
+    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
+    void *lw = rsvdt_scratch(SCR_G, rsvdt_hip_bshift_solve_work_bytes(g, Ns1, Nwe));
+    if (!work || !lw) return rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work");
+    const double *C1 = c->dC, *Ct = c->dC + 2 * (size_t)(kb * Ns1 * Nwe);
+#define GCHK(x) do { int _h = (x); if (_h) return rsvdt_err(_h, "GalerkinProj", "device call failed: %s", rsvdt_hip_last_error()); } while (0)
+    /* W = V^H X;  B = W Ct = V^H (ph y1 - y2) */
+    GCHK(rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, xs, Ns1, 1, g_trg.X, xs, Ns1, 1, c->W, work, q));
+    GCHK(rsvdt_hip_bapply(g, Ns1, Ns1, Nwe, c->W, Ns1 * Ns1, Ns1, 1, Ct, 0.0, NULL, 0, 0, 0, c->B, Ns1 * Nwe, Nwe, 1, q));
+    /* a = (ph I - M_tilde)^{-1} b, in place in B */
+    GCHK(rsvdt_hip_bshift_solve(g, Ns1, Nwe, TSR->M_tilde_all->d + 2 * (size_t)b0 * Ns1 * Ns1,
+                                c->dph + 2 * (size_t)b0 * Nwe, c->B, c->B, lw, c->info, q));
+    GCHK(rsvdt_hip_zscale((int64_t)g * Ns1 * Nwe, -1.0, c->B, c->B, q));
+    /* Y_hat = X C1 - V a  (per-test-vector rows), then to frequency blocks */
+    GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.X, xs, Ns1, 1, C1, 0.0, NULL, 0, 0, 0, c->Ybrm, N * Nwe, Nwe, 1, q));
+    GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.V, xs, Ns1, 1, c->B, 1.0, c->Ybrm, N * Nwe, Nwe, 1, c->Ybrm, N * Nwe,
+                          Nwe, 1, q));
+    GCHK(rsvdt_hip_transpose_rows(N, Nwe, c->k, b0, g, c->Ybrm, c->Yup->d, 1, q));
+#undef GCHK
+    trt_add(TRT_GALERKIN, t0);
+    return 0;
+}
+
+/* Ends the projection: on success Y_hat_up takes the result and M_tilde_all
+ * and Y_all are destroyed (GalerkinProjAllModes.c:41-42). */
+static PetscErrorCode gal_end(gal_ctx *c, PetscErrorCode ierr, Mat Yall, TS_removal_matrices *TSR)
+{
+    int hinfo = 0;
+    if (!ierr && rsvdt_hip_memcpy_d2h(&hinfo, c->info, sizeof(int)))
+        ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
+    if (hinfo) rsvdt_printf("[rsvdt] warning: %d singular Galerkin systems\n", hinfo);
+    gal_free(c);
+    tr_group_free(&g_trg);
+    if (ierr) { RSVDT_MatDestroy(&c->Yup); return ierr; }
+    RSVDT_MatDestroy(&TSR->M_tilde_all);      /* GalerkinProjAllModes.c:41 */
+    TSR->Y_hat_up = c->Yup;                    /* k-major, as the reference leaves it */
+    c->Yup = NULL;
+    RSVDT_MatDestroy(&Yall);                   /* GalerkinProjAllModes.c:42 */
+    return 0;
+}
+
+PetscErrorCode GalerkinProjAllModes(Mat Yall, LNS_vars *LNS, RSVDt_vars *V, TS_removal_matrices *TSR, DFT_matrices *D)
+{
+    (void)LNS;
+    if (!TSR->M_tilde_all) return rsvdt_err(RSVDT_ERR_USER, "GalerkinProjAllModes", "M_tilde_all is not set");
+    const PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1;
+    const PetscInt kb = tr_group_size(k, Yall->rows, Ns1);
+    gal_ctx c;
+    PetscErrorCode ierr = gal_begin(&c, Yall, V, D, kb);
+    if (ierr) { tr_group_free(&g_trg); return ierr; }
     for (PetscInt b0 = 0; b0 < k && !ierr; b0 += kb) {
         const PetscInt g = kb < k - b0 ? kb : k - b0;
         ierr = tr_group_build(Yall, k, b0, g, &g_trg);
-        if (ierr) break;
-        const int64_t xs = N * Ns1;
-        /* work sizes follow the group's own batch count */
-        void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(g, N, Ns1, Ns1));
-        void *lw = rsvdt_scratch(SCR_G, rsvdt_hip_bshift_solve_work_bytes(g, Ns1, Nwe));
-        if (!work || !lw) { ierr = rsvdt_err(RSVDT_ERR_MEM, "GalerkinProj", "cannot allocate work"); break; }
-#define GCHK(x) do { int _h = (x); if (_h) { ierr = rsvdt_err(_h, "GalerkinProj", "device call failed: %s", rsvdt_hip_last_error()); } } while (0)
-        /* W = V^H X;  B = W Ct = V^H (ph y1 - y2) */
-        GCHK(rsvdt_hip_bgram(g, N, Ns1, Ns1, g_trg.V, xs, Ns1, 1, g_trg.X, xs, Ns1, 1, W, work, q));
-        if (!ierr) GCHK(rsvdt_hip_bapply(g, Ns1, Ns1, Nwe, W, Ns1 * Ns1, Ns1, 1, Ct, 0.0, NULL, 0, 0, 0, B, Ns1 * Nwe,
-                                         Nwe, 1, q));
-        /* a = (ph I - M_tilde)^{-1} b, in place in B */
-        if (!ierr) GCHK(rsvdt_hip_bshift_solve(g, Ns1, Nwe, TSR->M_tilde_all->d + 2 * (size_t)b0 * Ns1 * Ns1,
-                                               dph + 2 * (size_t)b0 * Nwe, B, B, lw, info, q));
-        if (!ierr) GCHK(rsvdt_hip_zscale((int64_t)g * Ns1 * Nwe, -1.0, B, B, q));
-        /* Y_hat = X C1 - V a  (per-test-vector rows), then to frequency blocks */
-        if (!ierr) GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.X, xs, Ns1, 1, C1, 0.0, NULL, 0, 0, 0, Ybrm, N * Nwe, Nwe,
-                                         1, q));
-        if (!ierr) GCHK(rsvdt_hip_bapply(g, N, Ns1, Nwe, g_trg.V, xs, Ns1, 1, B, 1.0, Ybrm, N * Nwe, Nwe, 1, Ybrm,
-                                         N * Nwe, Nwe, 1, q));
-        if (!ierr) GCHK(rsvdt_hip_transpose_rows(N, Nwe, k, b0, g, Ybrm, Yup->d, 1, q));
-#undef GCHK
+        if (!ierr) ierr = gal_group(&c, TSR, b0, g);
     }
-    int hinfo = 0;
-    if (!ierr && rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)))
-        ierr = rsvdt_err(RSVDT_HIP_ERR_RUNTIME, "GalerkinProj", "%s", rsvdt_hip_last_error());
-    if (hinfo) rsvdt_printf("[rsvdt] warning: %d singular Galerkin systems\n", hinfo);
-    rsvdt_hip_sync();
-    if (dC) rsvdt_hip_free(dC);
-    if (dph) rsvdt_hip_free(dph);
-    if (Ybrm) rsvdt_hip_free(Ybrm);
-    tr_group_free(&g_trg);
-    if (ierr) { RSVDT_MatDestroy(&Yup); return ierr; }
-    RSVDT_MatDestroy(&TSR->M_tilde_all);      /* GalerkinProjAllModes.c:41 */
-    TSR->Y_hat_up = Yup;                       /* k-major, as the reference leaves it */
-    RSVDT_MatDestroy(&Yall);                   /* GalerkinProjAllModes.c:42 */
-    return 0;
+    return gal_end(&c, ierr, Yall, TSR);
 }
 
 /* SourceCode/TransientRemovalStrategy.c:30-40 */
@@ -990,11 +1087,33 @@ PetscErrorCode TransientRemovalStrategy(Mat Yall, LNS_vars *LNS, RSVD_matrices *
     double t1 = RSVDT_Time();
     if (V->Display > 0)
         rsvdt_printf("*** Trasnient removal strategy begins! ***\nThe process takes about as long as one period integration\n");
-    RSVDT_CHK(CreateMtilde(Yall, LNS, V, TSR));
-    RSVDT_CHK(GalerkinProjAllModes(Yall, LNS, V, TSR, D));
+    memset(g_trt, 0, sizeof(g_trt));
+    /* CreateMtilde and GalerkinProjAllModes group by group, so that each
+     * group's snapshot basis is built once (the reference computes the same
+     * QR in both, CreateMtilde.c:43-46 and GalerkinProjAllModes.c:31-33) */
+    RSVDT_CHK(mtilde_alloc(Yall, LNS, V, TSR));
+    const PetscInt k = V->RSVD.k, Ns1 = V->RSVD.Nw + 1, N = Yall->rows;
+    const PetscInt kb = tr_group_size(k, N, Ns1);
+    g_trt_kb = kb;
+    gal_ctx c;
+    PetscErrorCode ierr = gal_begin(&c, Yall, V, D, kb);
+    if (ierr) { tr_group_free(&g_trg); return ierr; }
+    for (PetscInt b0 = 0; b0 < k && !ierr; b0 += kb) {
+        const PetscInt g = kb < k - b0 ? kb : k - b0;
+        ierr = tr_group_build(Yall, k, b0, g, &g_trg);
+        if (!ierr) ierr = mtilde_group(LNS, V, TSR, N, Ns1, b0, g);
+        if (!ierr) ierr = gal_group(&c, TSR, b0, g);
+    }
+    RSVDT_CHK(gal_end(&c, ierr, Yall, TSR));
     RSVDT_CHK(ReversePermuteMat(TSR->Y_hat_up, V));
     R->Y_hat = TSR->Y_hat_up;
     TSR->Y_hat_up = NULL;
+    if (g_rsvdt.opt.verbose)
+        rsvdt_printf("[rsvdt] transient removal: %lld test vectors per group; basis %.1f ms, e^{A dT} steps %.1f ms, "
+                     "M_tilde %.1f ms, Galerkin %.1f ms (of %.1f ms)\n",
+                     (long long)g_trt_kb,
+                     g_trt[TRT_BASIS], g_trt[TRT_STEPS], g_trt[TRT_MTILDE], g_trt[TRT_GALERKIN],
+                     1e3 * (RSVDT_Time() - t1));
     if (V->Display > 0) rsvdt_printf("Trasnient removal strategy elapsed time = %.3f s\n", RSVDT_Time() - t1);
     return 0;
 }

@@ -131,5 +131,7 @@ PetscErrorCode rsvdt_svd_left(PetscInt nbatch, PetscInt nrows, PetscInt n, doubl
 #define RSVDT_NSCRATCH 32
 void *rsvdt_scratch(int slot, size_t bytes);
 void rsvdt_scratch_free_all(void);
+size_t rsvdt_scratch_size(int slot);     /* bytes held by a slot (0 if none) */
+void rsvdt_scratch_release(int slot);    /* free one slot's buffer */
 
 #endif

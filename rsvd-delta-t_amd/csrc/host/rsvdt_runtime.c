@@ -202,6 +202,7 @@ static void default_options(RSVDT_Options *o)
     o->rk4_form = 0;
     o->fuse_small = 1;
     o->spmm_variant = -1;
+    o->tr_group = 0;
 }
 
 PetscErrorCode RSVDT_Initialize(int *argc, char ***argv)
@@ -310,6 +311,19 @@ void *rsvdt_scratch(int slot, size_t bytes)
     if (rsvdt_hip_malloc(&g_scr[slot], bytes)) return NULL;
     g_scr_sz[slot] = bytes;
     return g_scr[slot];
+}
+
+void rsvdt_scratch_release(int slot)
+{
+    if (slot < 0 || slot >= NSCRATCH || !g_scr[slot]) return;
+    rsvdt_hip_sync();
+    rsvdt_hip_free(g_scr[slot]);
+    g_scr[slot] = NULL; g_scr_sz[slot] = 0;
+}
+
+size_t rsvdt_scratch_size(int slot)
+{
+    return slot >= 0 && slot < NSCRATCH && g_scr[slot] ? g_scr_sz[slot] : 0;
 }
 
 void rsvdt_scratch_free_all(void)

@@ -143,6 +143,7 @@ PetscErrorCode ReadUserInput(RSVDt_vars *RSVDt, Weight_matrices *W, LNS_vars *LN
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("UseGraph", &b, &ok) && ok) o.use_graph = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("FuseSmall", &b, &ok) && ok) o.fuse_small = b; }
     { int ok; PetscInt v; if (rsvdt_opt_get_int("SpMMVariant", &v, &ok) && ok) o.spmm_variant = (int)v; }
+    { int ok; PetscInt v; if (rsvdt_opt_get_int("TRGroup", &v, &ok) && ok) o.tr_group = (int)(v > 0 ? v : 0); }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Autotune", &b, &ok) && ok) o.autotune = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("XCDRemap", &b, &ok) && ok) o.xcd_remap = b; }
     { int ok; PetscBool b; if (rsvdt_opt_get_bool("Verbose", &b, &ok) && ok) o.verbose = b; }

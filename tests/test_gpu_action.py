@@ -48,6 +48,61 @@ def test_action_vs_oracle(pkg, N, Nw, cplx, tr, TL, k, direct, form, fuse, varia
     assert rel(Yg, Yo) < 1e-10
 
 
+@pytest.mark.parametrize("direct", [True, False])
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_removal_groups(pkg, direct, fuse):
+    """The transient removal in groups of test vectors (3, 3, 3, 1 of k = 10,
+    as a large operator is processed): the group buffers are reused across
+    groups and the basis is rebuilt for the Galerkin step."""
+    print(f"removal groups direct={direct} fuse={fuse}", flush=True)
+    N, Nw, k = 64, 16, 10
+    A = kat.gl_operator(N=N, L=20.0)
+    rng = np.random.default_rng(321)
+    o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=1.0, dt=0.02, transient_removal=True)
+    F = crandn(rng, N, k * o.Nw_eff)
+    Yo = o.action(F, direct=direct)
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=1.0, dt=0.02, transient_removal=True, fuse_small=fuse, tr_group=3)
+    s.set_sketch(F)
+    s.direct() if direct else s.adjoint()
+    Yg = s.get_Y()
+    s.close()
+    pkg.set_options(tr_group=0)
+    assert rel(Yg, Yo) < 1e-10
+
+
+@pytest.mark.parametrize("direct", [True, False])
+@pytest.mark.parametrize("group", [0, 3])
+def test_removal_entry_points(pkg, direct, group):
+    """CreateMtilde then GalerkinProjAllModes called through the C ABI on the
+    oracle's snapshots (SourceCode/TransientRemovalStrategy.c:30-36 without
+    the final permutation): Y_hat_up, k-major, against the oracle's action."""
+    import ctypes
+    print(f"removal entry points direct={direct} group={group}", flush=True)
+    N, Nw, k = 64, 16, 7
+    A = kat.gl_operator(N=N, L=20.0)
+    rng = np.random.default_rng(99)
+    o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=0.3, TL=1.0, dt=0.02, transient_removal=True)
+    F = crandn(rng, N, k * o.Nw_eff)
+    Ysnap = o.snapshots(F, direct=direct)
+    Yo = o.action(F, direct=direct)
+    Nwe = o.Nw_eff
+    Yo_k = Yo.reshape(N, Nwe, k).transpose(0, 2, 1).reshape(N, k * Nwe)     # column ik*Nwe + m
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=0.3, TL=1.0, dt=0.02, transient_removal=True, tr_group=group)
+    s.V.TS.DirAdj = int(direct)
+    L = pkg.lib()
+    Ym = pkg.mat_from_dense(np.asarray(Ysnap))
+    pkg.check(L.CreateMtilde(Ym, ctypes.byref(s.LNS), ctypes.byref(s.V), ctypes.byref(s.TSR)), "CreateMtilde")
+    pkg.check(L.GalerkinProjAllModes(Ym, ctypes.byref(s.LNS), ctypes.byref(s.V), ctypes.byref(s.TSR),
+                                     ctypes.byref(s.D)), "GalerkinProjAllModes")       # destroys Y_all
+    assert not s.TSR.M_tilde_all                                                     # destroyed as well
+    Yup = pkg.mat_to_dense(ctypes.c_void_p(s.TSR.Y_hat_up))
+    pkg.destroy(s.TSR.Y_hat_up)
+    s.TSR.Y_hat_up = None
+    s.close()
+    pkg.set_options(tr_group=0)
+    assert rel(Yup, Yo_k) < 1e-10
+
+
 def test_chunk_boundaries_and_lanes(pkg):
     """Forcing-chunk sizes that do not divide the step count, and every lane width."""
     A = kat.gl_operator(N=48, L=20.0)
