@@ -165,6 +165,8 @@ def main():
     ap.add_argument("--no-ttm", action="store_true",
                     help="skip the time-to-modes comparison on held-out Ginzburg-Landau family members")
     ap.add_argument("--ttm-members", type=int, default=4)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each forcing chunk's stage launches from a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design shards test vectors "
@@ -195,7 +197,7 @@ def main():
     TL = T / 2
     s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, precision=args.precision,
                     chunk_steps=args.chunk, overlap_forcing=0 if args.no_overlap else 1, verbose=1,
-                    rk4_form=args.rk4_form)
+                    rk4_form=args.rk4_form, use_graph=int(args.graph))
     s.random_sketch(seed=1373, gaussian=True)
     ms, rk_ms, fr_ms = s.bench_steps(args.steps, warmup=args.warmup)
     st = pkg.action_stats()
@@ -249,6 +251,7 @@ def main():
         "config": {"workload": f"{wl} RK4 time stepping, k={k}, Nw={Nw} (Nw_eff={Nwe}), transient removal, "
                                + ("FP64" if args.precision == 64 else "FP32 state"),
                    "N": N, "k": k, "Nw": Nw, "dt": dt, "chunk_steps": S, "precision": args.precision,
+                   "hip_graph": bool(args.graph),
                    "rk4_form": "horner+folded forcing" if form == 0 else "reference stage order",
                    "bytes_per_step_reference_form": int(bytes_step_ref),
                    "bytes_per_step_algorithmic": int(bytes_step_alg),
