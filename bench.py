@@ -61,7 +61,40 @@ def cpu_baseline(A, k, Nw, w, TL, dt, bytes_per_step, budget_s=12.0):
     return {"value": round(value, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{nsteps} forced RK4 steps of 1 test vector (4 SpMV + 2 forcing GEMV + 7 AXPY each, "
                       f"as SourceCode/TSRK4.c), {per_vector_step * 1e3:.1f} ms per vector-step; "
-                      f"scaled to k={k} vectors per block step"}
+                      f"scaled to k={k} vectors per block step",
+            "vector_step_ms": round(per_vector_step * 1e3, 4)}
+
+
+def c0_full_run(pkg):
+    """The whole main.c sequence on the CPU-runnable Ginzburg-Landau case
+    (N=500, U=2+0.2i, k=10, Nw=64, w=0.05, dt=T/4096, TL=10, transient
+    removal, q=0) with the same seeded F_hat: the CPU oracle on one core
+    against the engine's default form on the GPU (second run, after the
+    autotuner), and the largest relative difference of their gains."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import workloads
+    A = workloads.gl(500)
+    k, Nw, w, TL = 10, 64, 0.05, 10.0
+    dt = 2 * np.pi / w / 4096
+    rng = np.random.default_rng(2024)
+    o = oracle.Oracle(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True, nthreads=1)
+    F = rng.standard_normal((500, k * o.Nw_eff)) + 1j * rng.standard_normal((500, k * o.Nw_eff))
+    t0 = time.time()
+    _, _, S_cpu = o.run(F)
+    t_cpu = time.time() - t0
+    s = pkg.Session(A, k=k, q=0, Nw=Nw, w=w, TL=TL, dt=dt, transient_removal=True)
+    s.run(F)
+    pkg.lib().rsvdt_hip_sync()
+    t0 = time.time()
+    _, _, S_gpu = s.run(F)
+    pkg.lib().rsvdt_hip_sync()
+    t_gpu = time.time() - t0
+    s.close()
+    return {"workload": "Ginzburg-Landau N=500, k=10, Nw=64, TL=10, transient removal, q=0 (main.c sequence)",
+            "cpu_oracle_s": round(t_cpu, 3), "cpu_cores": 1, "gpu_s": round(t_gpu, 4),
+            "speedup": round(t_cpu / t_gpu, 1),
+            "sigma_max_rel_diff": float(np.max(np.abs(S_gpu - S_cpu) / np.abs(S_cpu)))}
 
 
 def calib_copy(pkg, gib=1, reps=5):
@@ -264,8 +297,17 @@ def main():
                      "avg_launch_ms": round(launch_ms, 5)},
         "breakdown_ms_per_step": {"rk4": round(rk_ms / args.steps, 5), "forcing": round(fr_ms / args.steps, 5)},
     }
+    # one action's time stepping: Nt + Ns + RR steps with the transient removal
+    rend = s.Nt + s.Ns + s.RR
+    out["action_stepping_s"] = round(ms_step * rend / 1e3, 3)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(A, k, Nw, w, TL, dt, bytes_step_ref)
+        cb = cpu_baseline(A, k, Nw, w, TL, dt, bytes_step_ref)
+        # the same action's stepping on the host cores, extrapolated from the
+        # sample (k test vectors, one after the other as TSActionRK4.c:89 does)
+        cb["action_stepping_s_extrapolated"] = round(cb["vector_step_ms"] * 1e-3 * k * rend, 1)
+        if args.workload == "flow2d":
+            cb["ginzburg_landau_full_run"] = c0_full_run(pkg)
+        out["cpu_baseline"] = cb
     if args.full:
         rng = None
         res = {}
