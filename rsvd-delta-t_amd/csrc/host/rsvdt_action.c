@@ -819,6 +819,8 @@ static PetscInt tr_group_size(PetscInt k, PetscInt N, PetscInt n)
     return p2;
 }
 
+void rsvdt_action_release(void) { tr_group_free(&g_trg); }
+
 /* Snapshots (relaid) and orthonormal basis of test vectors b0 .. b0+kb-1. */
 static PetscErrorCode tr_group_build(Mat Yall, PetscInt k, PetscInt b0, PetscInt kb, tr_group *g)
 {

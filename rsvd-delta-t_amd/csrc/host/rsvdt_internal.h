@@ -132,6 +132,7 @@ PetscErrorCode rsvdt_svd_left(PetscInt nbatch, PetscInt nrows, PetscInt n, doubl
 void *rsvdt_scratch(int slot, size_t bytes);
 void rsvdt_scratch_free_all(void);
 size_t rsvdt_scratch_size(int slot);     /* bytes held by a slot (0 if none) */
+void rsvdt_action_release(void);         /* device buffers the transient removal keeps between calls */
 void rsvdt_scratch_release(int slot);    /* free one slot's buffer */
 
 #endif

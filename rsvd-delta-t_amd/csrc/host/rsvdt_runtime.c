@@ -247,6 +247,7 @@ PetscErrorCode rsvdt_ensure_device(void)
 
 PetscErrorCode RSVDT_Finalize(void)
 {
+    if (g_rsvdt.device >= 0) rsvdt_action_release();
     rsvdt_scratch_free_all();
     if (g_rsvdt.device >= 0) {
         rsvdt_hip_stream_destroy(g_rsvdt.stream);
