@@ -85,9 +85,9 @@ typedef struct {
                              2 / 3 chunks of 64 / 256 row blocks dealt round-robin to the XCDs */
     int variant;          /* 0: 16-nonzero gather batches; 1: + nontemporal row-local
                              streams; 2: 8-nonzero batches; 3: 8 + nontemporal; 4: 12-nonzero
-                             batches; 5: 12 + nontemporal (variants 1-3 apply when k <= lanes,
-                             in the Horner stage when k <= 2 lanes; 4 and 5 only to the Horner
-                             stage with lanes >= 16 and k <= 2 lanes); 6: the
+                             batches; 5: 12 + nontemporal (variants 1-3 apply to the RK4 stage
+                             when k <= lanes and to every Horner stage; 4 and 5 only to the
+                             Horner stage with lanes >= 16 and k <= 2 lanes); 6: the
                              Horner stage on the row-grouped operator below (k <= lanes,
                              16..64 lanes); 7: the FP32 Horner stage with two adjacent
                              columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes) */

@@ -722,9 +722,9 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
             if (var == 4) return launch_horner<T, AREAL, L, C, 12, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);     \
             if (var == 5) return launch_horner<T, AREAL, L, C, 12, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);      \
         }                                                                                                         \
-        if (C <= 2 && var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
-        if (C <= 2 && var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s); \
-        if (C <= 2 && var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);  \
+        if (var == 1) return launch_horner<T, AREAL, L, C, 16, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);          \
+        if (var == 2) return launch_horner<T, AREAL, L, C, 8, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);          \
+        if (var == 3) return launch_horner<T, AREAL, L, C, 8, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);           \
         return launch_horner<T, AREAL, L, C, 16, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);                        \
     }
     RSVDT_HCASE(4, 1) RSVDT_HCASE(8, 1) RSVDT_HCASE(16, 1) RSVDT_HCASE(32, 1) RSVDT_HCASE(64, 1)

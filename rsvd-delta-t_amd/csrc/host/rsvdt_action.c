@@ -352,13 +352,14 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
         if ((k + l - 1) / l > 4) continue;            /* at most 4 columns per lane */
         if (l >= 2 * k && l > 4) continue;            /* no more than half the lanes idle */
         const int cpl = (int)((k + l - 1) / l);
-        /* gather-batch variants: 0-3 for one column per lane (and two in the
-         * Horner stage), 4-5 (12-nonzero batches) with 16 or more lanes, 6-7
-         * (grouped rows, FP32 pairs) with one column per lane */
+        /* gather-batch variants: 0-3 (the RK4 stage: with one column per
+         * lane), 4-5 (12-nonzero batches) with 16 or more lanes and at most
+         * two columns per lane, 6-7 (grouped rows, FP32 pairs) with one */
         int nvar = 1;
         if (f) nvar = cpl > 1 ? 1 : 4;
         else if (cpl == 1) nvar = l >= 16 ? 8 : 4;
         else if (cpl == 2) nvar = l >= 16 ? 6 : 4;
+        else nvar = 4;
         for (int v = 0; v < nvar; v++)
             for (int r = 0; r < 4; r++) {
                 dsc.lanes = l; dsc.variant = v; dsc.xcd_remap = r;
