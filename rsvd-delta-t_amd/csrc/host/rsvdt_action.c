@@ -95,6 +95,27 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
         d->variant = 3;                                                         /* no column pairs */
 }
 
+/* Launch shape of the stages that run on several blocks at once (the folded
+ * forcing terms, the e^{A dT} steps of the transient removal), which the
+ * autotuner does not see: two columns per lane (half of k rounded up to a
+ * power of two, at least 4 lanes) with 8-nonzero batches, the shape it picks
+ * for the 16- and 32-column blocks of the flow operators.  An explicit
+ * SpMMVariant option keeps its variant with the default lanes (the grouped
+ * and column-pair variants fix their own); an explicit SpMMLanes is kept. */
+static void multi_block_shape(rsvdt_spmm_desc *d, PetscInt k)
+{
+    d->xcd_remap = 0;
+    if (g_rsvdt.opt.spmm_variant >= 0) {
+        d->lanes = 0;
+        return;
+    }
+    d->variant = 3;
+    if (g_rsvdt.opt.spmm_lanes) return;
+    int l = 4;
+    while (l < k && l < 64) l <<= 1;
+    d->lanes = l / 2 >= 4 ? l / 2 : 4;
+}
+
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
  * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
  * while the two chunk buffers would take more than 60 % of free memory (an
@@ -274,7 +295,7 @@ static PetscErrorCode folded_forcing(Mat A, int dir, double h, PetscInt k, Petsc
         const PetscInt g = grp < Nwe - b0 ? grp : Nwe - b0;
         rsvdt_spmm_desc d;
         desc_for(&d, A, dir, 64, k, g);
-        d.lanes = 0;
+        multi_block_shape(&d, k);
         const double *Fb = F + 2 * (size_t)b0 * (size_t)N * (size_t)k;
         double *Gb = G + 2 * (size_t)b0 * (size_t)N * (size_t)k;
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, Fb, Fb, dsc + 2 * b0, NULL, h / 4, w1, NULL, q));
@@ -548,7 +569,7 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         PetscInt g = G < nb - g0 ? G : nb - g0;
         rsvdt_spmm_desc d;
         desc_for(&d, A, dir, 64, k, g);
-        d.lanes = 0;
+        multi_block_shape(&d, k);
         double *y = X + 2 * (size_t)g0 * (size_t)N * (size_t)k;
         rsvdt_stream_t q = g_rsvdt.stream;
         if (fused) {
