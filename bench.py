@@ -256,7 +256,7 @@ def main():
     # what the form that ran needs per step
     if form == 0:
         bytes_step_alg = 4 * S_A + 12 * V + (Nwe / S + 1) * V
-        kernel = {6: "k_horner_group", 7: "k_horner_pair"}.get(st["spmm_variant"], "k_horner_stage")
+        kernel = {6: "k_horner_group", 7: "k_horner_pair", 8: "k_horner_pair"}.get(st["spmm_variant"], "k_horner_stage")
         bytes_launch = S_A + 3 * V
     else:
         bytes_step_alg = 4 * S_A + 20 * V + (2 * Nwe / (2 * S + 1) + 2) * V

@@ -90,7 +90,8 @@ typedef struct {
                              Horner stage with lanes >= 16 and k <= 2 lanes); 6: the
                              Horner stage on the row-grouped operator below (k <= lanes,
                              16..64 lanes); 7: the FP32 Horner stage with two adjacent
-                             columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes) */
+                             columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes);
+                             8: the same with two such pairs per lane (k/4 in 4, 8, 16 lanes) */
     /* row-grouped form of the same operator (variant 6): groups of `group`
      * consecutive rows (2, 3, 4, 5 or 8); group q's distinct columns are
      * gword[gptr[q] .. gptr[q+1]-1], each (column << 8 | mask of the group's

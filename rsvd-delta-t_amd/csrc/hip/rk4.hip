@@ -572,10 +572,11 @@ static int dispatch_horner_group(const rsvdt_spmm_desc *d, const void *vin, cons
 // instead of two 8-byte ones: half the load instructions per gathered byte.
 // LPR = k/2 lanes per row (k even, k/2 a power of two), 8-nonzero batches,
 // nontemporal row-local streams.
-template <bool AREAL, int LPR, int NB>
+template <bool AREAL, int LPR, int NB, int PPL>
 __device__ __forceinline__ void gather_batch_pair(int eb, int e1, int lane, int k, const int32_t *__restrict__ colidx,
                                                   const typename aval<float, AREAL>::type *__restrict__ vals,
-                                                  const float4 *__restrict__ ub4, double (&sr)[2], double (&si)[2])
+                                                  const float4 *__restrict__ ub4, double (&sr)[2 * PPL],
+                                                  double (&si)[2 * PPL])
 {
     using AV = typename aval<float, AREAL>::type;
     constexpr int EPL = LPR >= NB ? 1 : NB / LPR;
@@ -598,25 +599,30 @@ __device__ __forceinline__ void gather_batch_pair(int eb, int e1, int lane, int 
         as[u] = shfl_val(mv[u / LPR < EPL ? u / LPR : 0], u % LPR, LPR);
     }
     const int k2 = k >> 1;
-    float4 xs[NB];
+    float4 xs[NB][PPL];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) xs[u] = ub4[(int64_t)cs[u] * k2 + lane];
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int p = 0; p < PPL; ++p) xs[u][p] = ub4[(int64_t)cs[u] * k2 + lane + p * LPR];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         if (u < cnt) {
-            const double x0r = xs[u].x, x0i = xs[u].y, x1r = xs[u].z, x1i = xs[u].w;
-            if constexpr (AREAL) {
-                sr[0] += as[u] * x0r; si[0] += as[u] * x0i;
-                sr[1] += as[u] * x1r; si[1] += as[u] * x1i;
-            } else {
-                sr[0] += as[u].x * x0r - as[u].y * x0i; si[0] += as[u].x * x0i + as[u].y * x0r;
-                sr[1] += as[u].x * x1r - as[u].y * x1i; si[1] += as[u].x * x1i + as[u].y * x1r;
+#pragma unroll
+            for (int p = 0; p < PPL; ++p) {
+                const double x0r = xs[u][p].x, x0i = xs[u][p].y, x1r = xs[u][p].z, x1i = xs[u][p].w;
+                if constexpr (AREAL) {
+                    sr[2 * p] += as[u] * x0r; si[2 * p] += as[u] * x0i;
+                    sr[2 * p + 1] += as[u] * x1r; si[2 * p + 1] += as[u] * x1i;
+                } else {
+                    sr[2 * p] += as[u].x * x0r - as[u].y * x0i; si[2 * p] += as[u].x * x0i + as[u].y * x0r;
+                    sr[2 * p + 1] += as[u].x * x1r - as[u].y * x1i; si[2 * p + 1] += as[u].x * x1i + as[u].y * x1r;
+                }
             }
         }
     }
 }
 
-template <bool AREAL, int LPR>
+template <bool AREAL, int LPR, int PPL>
 __global__ __launch_bounds__(256) void k_horner_pair(
     int nrows, int k, int xcd_remap, int64_t bstride, const int32_t *__restrict__ rowptr,
     const int32_t *__restrict__ colidx, const void *__restrict__ vals_, const float2 *__restrict__ vin,
@@ -636,40 +642,53 @@ __global__ __launch_bounds__(256) void k_horner_pair(
         const int sl = *snap_slot;
         snap = sl >= 0 ? snap + (int64_t)sl * snap_stride : nullptr;
     }
-    const int64_t idx = boff + (int64_t)row * k + 2 * lane;     // first of the lane's two columns
+    // pair p of the lane: columns 2 (lane + p LPR) and 2 (lane + p LPR) + 1
+    const int64_t idx0 = boff + (int64_t)row * k + 2 * lane;
     const bool nty = (const void *)yv != (const void *)vin;
     typedef float ev4f __attribute__((ext_vector_type(4)));
-    ev4f yv4 = nty ? __builtin_nontemporal_load((const ev4f *)(yv + idx)) : *(const ev4f *)(yv + idx);
-    ev4f gv4 = {0.f, 0.f, 0.f, 0.f};
-    if (g != nullptr) gv4 = __builtin_nontemporal_load((const ev4f *)(g + idx));
-    const float4 yr = make_float4(yv4.x, yv4.y, yv4.z, yv4.w), gr = make_float4(gv4.x, gv4.y, gv4.z, gv4.w);
-    double sr[2] = {0, 0}, si[2] = {0, 0};
+    float4 yr[PPL], gr[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+        const int64_t idx = idx0 + 2 * p * LPR;
+        const ev4f yv4 = nty ? __builtin_nontemporal_load((const ev4f *)(yv + idx)) : *(const ev4f *)(yv + idx);
+        ev4f gv4 = {0.f, 0.f, 0.f, 0.f};
+        if (g != nullptr) gv4 = __builtin_nontemporal_load((const ev4f *)(g + idx));
+        yr[p] = make_float4(yv4.x, yv4.y, yv4.z, yv4.w);
+        gr[p] = make_float4(gv4.x, gv4.y, gv4.z, gv4.w);
+    }
+    double sr[2 * PPL], si[2 * PPL];
+#pragma unroll
+    for (int t = 0; t < 2 * PPL; ++t) { sr[t] = 0; si[t] = 0; }
     const float4 *ub4 = (const float4 *)(vin + boff);
     int eb = e0;
-    for (; e1 - eb >= NB; eb += NB) gather_batch_pair<AREAL, LPR, NB>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    for (; e1 - eb >= NB; eb += NB) gather_batch_pair<AREAL, LPR, NB, PPL>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
     const int rem = e1 - eb;
-    if (rem > NB / 2) gather_batch_pair<AREAL, LPR, NB>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
-    else if (rem > NB / 4) gather_batch_pair<AREAL, LPR, NB / 2>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
-    else if (rem > 0) gather_batch_pair<AREAL, LPR, NB / 4>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
-    double y0r = yr.x, y0i = yr.y, y1r = yr.z, y1i = yr.w;
-    if (ys != nullptr) {
-        const double a = ys[blockIdx.y].x, b = ys[blockIdx.y].y;
-        double t = a * y0r - b * y0i; y0i = a * y0i + b * y0r; y0r = t;
-        t = a * y1r - b * y1i; y1i = a * y1i + b * y1r; y1r = t;
-    }
-    float4 o;
-    o.x = (float)(y0r + c * sr[0] + (double)gr.x);
-    o.y = (float)(y0i + c * si[0] + (double)gr.y);
-    o.z = (float)(y1r + c * sr[1] + (double)gr.z);
-    o.w = (float)(y1i + c * si[1] + (double)gr.w);
-    *(float4 *)(out + idx) = o;
-    if (snap != nullptr) {
-        st_stream(snap + idx, make_double2((double)o.x, (double)o.y), true);
-        st_stream(snap + idx + 1, make_double2((double)o.z, (double)o.w), true);
+    if (rem > NB / 2) gather_batch_pair<AREAL, LPR, NB, PPL>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    else if (rem > NB / 4) gather_batch_pair<AREAL, LPR, NB / 2, PPL>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+    else if (rem > 0) gather_batch_pair<AREAL, LPR, NB / 4, PPL>(eb, e1, lane, k, colidx, vals, ub4, sr, si);
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+        const int64_t idx = idx0 + 2 * p * LPR;
+        double y0r = yr[p].x, y0i = yr[p].y, y1r = yr[p].z, y1i = yr[p].w;
+        if (ys != nullptr) {
+            const double a = ys[blockIdx.y].x, b = ys[blockIdx.y].y;
+            double t = a * y0r - b * y0i; y0i = a * y0i + b * y0r; y0r = t;
+            t = a * y1r - b * y1i; y1i = a * y1i + b * y1r; y1r = t;
+        }
+        float4 o;
+        o.x = (float)(y0r + c * sr[2 * p] + (double)gr[p].x);
+        o.y = (float)(y0i + c * si[2 * p] + (double)gr[p].y);
+        o.z = (float)(y1r + c * sr[2 * p + 1] + (double)gr[p].z);
+        o.w = (float)(y1i + c * si[2 * p + 1] + (double)gr[p].w);
+        *(float4 *)(out + idx) = o;
+        if (snap != nullptr) {
+            st_stream(snap + idx, make_double2((double)o.x, (double)o.y), true);
+            st_stream(snap + idx + 1, make_double2((double)o.z, (double)o.w), true);
+        }
     }
 }
 
-template <bool AREAL, int L>
+template <bool AREAL, int L, int PPL = 1>
 static int launch_horner_pair(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
                               double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
@@ -677,7 +696,7 @@ static int launch_horner_pair(const rsvdt_spmm_desc *d, const void *vin, const v
     const unsigned gx = ceil_div(d->nrows, RPB);
     const unsigned gy = (unsigned)d->nb;
     const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
-    hipLaunchKernelGGL((k_horner_pair<AREAL, L>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows, (int)d->k, remap,
+    hipLaunchKernelGGL((k_horner_pair<AREAL, L, PPL>), dim3(gx, gy), dim3(256), 0, s, (int)d->nrows, (int)d->k, remap,
                        d->nrows * d->k, d->rowptr, d->col, d->val, (const float2 *)vin, (const float2 *)y,
                        (const double2 *)ys, (const float2 *)g, (float2 *)out, c, (double2 *)snap, slot, sstride);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage");
@@ -706,6 +725,18 @@ static int dispatch_horner(const rsvdt_spmm_desc *d, const void *vin, const void
             }
         }
         set_error_msg("rsvdt_hip_horner_stage: variant 7 needs the FP32 state and k/2 a power of two in 4..32");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    if (d->variant == 8) {
+        if constexpr (sizeof(T) == 4) {
+            const int64_t l = d->k / 4;
+            if (d->k % 4 == 0) {
+                if (l == 4) return launch_horner_pair<AREAL, 4, 2>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+                if (l == 8) return launch_horner_pair<AREAL, 8, 2>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+                if (l == 16) return launch_horner_pair<AREAL, 16, 2>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+            }
+        }
+        set_error_msg("rsvdt_hip_horner_stage: variant 8 needs the FP32 state and k/4 a power of two in 4..16");
         return RSVDT_HIP_ERR_ARG;
     }
     int lpr = d->lanes;

@@ -93,6 +93,8 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     if (d->variant == 7 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 2 || !(k / 2 == 4 || k / 2 == 8 ||
                                                                          k / 2 == 16 || k / 2 == 32)))
         d->variant = 3;                                                         /* no column pairs */
+    if (d->variant == 8 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 4 || !(k / 4 == 4 || k / 4 == 8 || k / 4 == 16)))
+        d->variant = 3;                                                         /* no double pairs */
 }
 
 /* Launch shape of the stages that run on several blocks at once (the folded
@@ -398,16 +400,20 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
                 if (ms < best) { best = ms; bl = l; bv = v; br = r; }
             }
     }
-    if (!f && prec == 32 && k % 2 == 0 && (k / 2 == 4 || k / 2 == 8 || k / 2 == 16 || k / 2 == 32)) {
-        for (int r = 0; r < 4; r++) {              /* FP32 column pairs (lanes = k/2) */
-            dsc.lanes = (int)(k / 2); dsc.variant = 7; dsc.xcd_remap = r;
+    /* FP32 column pairs: one pair per lane (variant 7, lanes = k/2) or two
+     * (variant 8, lanes = k/4) */
+    for (int v = 7; v <= 8 && !f && prec == 32; v++) {
+        const PetscInt per = v == 7 ? 2 : 4, l = k / per;
+        if (k % per || !(l == 4 || l == 8 || l == 16 || (v == 7 && l == 32))) continue;
+        for (int r = 0; r < 4; r++) {
+            dsc.lanes = (int)l; dsc.variant = v; dsc.xcd_remap = r;
             if (rsvdt_hip_horner_stage(&dsc, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream)) continue;
             rsvdt_hip_event_record(e0, g_rsvdt.stream);
             for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&dsc, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream);
             rsvdt_hip_event_record(e1, g_rsvdt.stream);
             float ms = 0;
             rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
-            if (ms < best) { best = ms; bl = (int)(k / 2); bv = 7; br = r; }
+            if (ms < best) { best = ms; bl = (int)l; bv = v; br = r; }
         }
     }
     rsvdt_hip_event_destroy(e0);

@@ -187,7 +187,8 @@ def test_rk4_step(L, pkg, k, real, prec, nb, lanes, variant):
     (64, False, 32, 1, 32, 7, False, True), (32, True, 64, 1, 16, 4, False, True), (32, False, 64, 2, 16, 5, True, True),
     (24, True, 64, 1, 16, 5, False, False), (32, True, 64, 1, 16, 2, False, True), (32, False, 64, 2, 16, 3, True, False),
     (16, True, 64, 1, 8, 1, False, True), (16, True, 64, 1, 4, 3, False, True), (32, False, 64, 2, 8, 2, True, True),
-    (13, True, 32, 1, 4, 1, False, False)])
+    (13, True, 32, 1, 4, 1, False, False), (16, True, 32, 1, 4, 8, False, True), (32, False, 32, 2, 8, 8, True, True),
+    (64, True, 32, 1, 16, 8, True, False)])
 def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced):
     """out_b = ys_b y_b + c A vin_b + g_b, with the snapshot copy, against numpy."""
     print(f"horner k={k} real={real} prec={prec} nb={nb} lanes={lanes} variant={variant} scaled={scaled} "
