@@ -91,7 +91,9 @@ typedef struct {
                              Horner stage on the row-grouped operator below (k <= lanes,
                              16..64 lanes); 7: the FP32 Horner stage with two adjacent
                              columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes);
-                             8: the same with two such pairs per lane (k/4 in 4, 8, 16 lanes) */
+                             8: the same with two such pairs per lane (k/4 in 4, 8, 16 lanes);
+                             9: the Horner stage on the tiled form below (lanes = columns per
+                             pass: 8, 16 or 32, 0 = automatic) */
     /* row-grouped form of the same operator (variant 6): groups of `group`
      * consecutive rows (2, 3, 4, 5 or 8); group q's distinct columns are
      * gword[gptr[q] .. gptr[q+1]-1], each (column << 8 | mask of the group's
@@ -102,6 +104,18 @@ typedef struct {
     const int32_t *gptr;
     const uint32_t *gword;
     const void *gval;
+    /* tiled form of the same operator (variant 9): tiles of tile_rows
+     * consecutive rows; tile t's distinct columns are tcol[tptr[t] ..
+     * tptr[t+1]-1], ascending, and lidx[e] is the position of nonzero e's
+     * column in its tile's list.  tile_hmax / tile_emax: the most columns /
+     * nonzeros of any tile.  A workgroup stages its tile's rows of the
+     * gathered vector in LDS once and forms every row of the tile from
+     * there. */
+    int tile_rows;
+    int tile_hmax, tile_emax;
+    const int32_t *tptr;
+    const int32_t *tcol;
+    const uint16_t *lidx;
 } rsvdt_spmm_desc;
 
 int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,

@@ -27,31 +27,6 @@
 
 namespace rsvdt {
 
-// Workgroup -> row-block map.  Dispatch hands workgroup i to XCD i % 8.
-// mode 0: identity.  mode 1: XCD x gets one contiguous range of row blocks
-// (its L2 sees the stencil neighbours, but the 8 XCDs sweep 8 distant parts
-// of the vector at once).  modes 2, 3: XCD x gets chunks x, x+8, x+16, ... of
-// 64 (mode 2) or 256 (mode 3) row blocks, so all XCDs sweep the same region
-// of the vector together (one reuse window in the shared last-level cache)
-// while each chunk's near neighbours stay in one L2; the tail that does not
-// fill a whole round of 8 chunks keeps the identity map.
-__device__ __forceinline__ int xcd_block(int bx, int nb, int mode)
-{
-    if (mode == 1) {
-        const int q = nb >> 3, r = nb & 7, x = bx & 7, idx = bx >> 3;
-        return x * q + (x < r ? x : r) + idx;
-    }
-    if (mode >= 2) {
-        const int ch = mode == 2 ? 64 : 256;
-        const int full = nb / (8 * ch);
-        if (bx < full * 8 * ch) {
-            const int x = bx & 7, idx = bx >> 3;
-            return ((idx / ch) * 8 + x) * ch + idx % ch;
-        }
-    }
-    return bx;
-}
-
 // Matrix values are FP64 for both state precisions (real or complex); the
 // SpMM accumulates in FP64 and only the stored state vectors are FP32 in the
 // FP32 form (rounding A itself to FP32 would perturb the operator).
@@ -1073,6 +1048,7 @@ extern "C" int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *v
         return RSVDT_HIP_ERR_ARG;
     }
     hipStream_t st = as_stream(s);
+    if (d->variant == 9) return horner_tile(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
     if (d->prec == 64) {
         if (d->a_real) return dispatch_horner<double, true>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
         return dispatch_horner<double, false>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);

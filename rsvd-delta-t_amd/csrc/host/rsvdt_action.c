@@ -59,6 +59,40 @@ typedef struct {
     const double *idft;          /* [2Ns][Nwe] FP64 complex */
 } stepper;
 
+static void stepper_free(stepper *st);
+
+
+/* Columns per pass of the tiled stage kernel: lanes, or by default 8 for
+ * blocks of at most 8 columns and 16 otherwise; the tile heights its launch
+ * shapes allow: (256 / columns per pass) x 1, 2, 3 or 4 rows per slot (32
+ * columns per pass: 1, 2, 4; 8 columns: 1). */
+static int tile_kc(const rsvdt_spmm_desc *d) { return d->lanes ? d->lanes : (d->k <= 8 ? 8 : 16); }
+static int tile_ok(int kc, int tr)
+{
+    if (kc == 8) return tr == 32;
+    if (kc == 16) return tr == 16 || tr == 32 || tr == 48 || tr == 64;
+    if (kc == 32) return tr == 8 || tr == 16 || tr == 32;
+    return 0;
+}
+
+/* Points the descriptor at the tiled form of A (dir 1) or A^H with tiles of
+ * `tile` rows (0: 32 rows, or 16 with 32 columns per pass), building it on
+ * first use.  Returns 0 when there is none (the caller falls back). */
+static int tile_attach(rsvdt_spmm_desc *d, Mat A, int dir, int tile)
+{
+    const int kc = tile_kc(d);
+    if (!tile) tile = kc == 32 ? 16 : 32;
+    if (!tile_ok(kc, tile)) return 0;
+    const rsvdt_tiled *t = NULL;
+    if (rsvdt_tiled_get(A, dir, tile, &t) || !t) return 0;
+    d->tile_rows = t->rows;
+    d->tile_hmax = t->hmax;
+    d->tile_emax = t->emax;
+    d->tptr = t->d_tptr;
+    d->tcol = t->d_tcol;
+    d->lidx = t->d_lidx;
+    return 1;
+}
 
 static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, PetscInt nb)
 {
@@ -76,10 +110,12 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
     d->gword = dir ? A->d_gword : A->d_gwordH;
     d->gval = dir ? A->d_gval : A->d_gvalH;
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
+    int tile = 0;
     if (A->tuned_ok[f][p][q] && A->tuned_k[f][p][q] == k && nb == 1) {
         d->lanes = A->tuned_lanes[f][p][q];
         d->variant = A->tuned_variant[f][p][q];
         d->xcd_remap = A->tuned_remap[f][p][q];
+        tile = A->tuned_tile[f][p][q];
     } else {
         /* measured best on the 2D flow operator: 8-nonzero gather batches with
          * nontemporal row-local streams, lanes = k */
@@ -95,6 +131,9 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
         d->variant = 3;                                                         /* no column pairs */
     if (d->variant == 8 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 4 || !(k / 4 == 4 || k / 4 == 8 || k / 4 == 16)))
         d->variant = 3;                                                         /* no double pairs */
+    if (d->variant == 9) {
+        if (g_rsvdt.opt.rk4_form || !tile_attach(d, A, dir, tile)) d->variant = 3;   /* no tiled form */
+    }
 }
 
 /* Launch shape of the stages that run on several blocks at once (the folded
@@ -160,11 +199,16 @@ static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double
     if (!st->y || !st->u1 || !st->u2 || !st->acc || !st->fbuf[0] || !st->fbuf[1])
         return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate the RK4 work buffers (%.2f GB each): %s",
                          blk / 1e9, rsvdt_hip_last_error());
-    for (int b = 0; b < 2; b++) {
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_create(&st->ev_frc[b]));
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_create(&st->ev_rk[b]));
+    int hr = 0;
+    for (int b = 0; b < 2 && !hr; b++) {
+        hr = rsvdt_hip_event_create(&st->ev_frc[b]);
+        if (!hr) hr = rsvdt_hip_event_create(&st->ev_rk[b]);
     }
-    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st->y, blk, g_rsvdt.stream));
+    if (!hr) hr = rsvdt_hip_zero(st->y, blk, g_rsvdt.stream);
+    if (hr) {
+        stepper_free(st);
+        return rsvdt_err(hr, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+    }
     return 0;
 }
 
@@ -358,6 +402,9 @@ static void pt_free(phase_timer *t)
  * variant and XCD row remap -- by timing three launches of every candidate
  * on scratch buffers (a Horner stage, or stage 2 of the stage form).  The
  * choice is kept on the Mat and reused by every later action. */
+/* (columns per pass, tile height) of the tiled kernel the autotuner times */
+static const int tile_cand[][2] = {{8, 32}, {16, 16}, {16, 32}, {32, 8}};
+
 static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void *a, void *b, void *c, void *d)
 {
     const int p = prec == 32, q = dir ? 0 : 1, f = g_rsvdt.opt.rk4_form ? 1 : 0;
@@ -416,16 +463,39 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
             if (ms < best) { best = ms; bl = (int)l; bv = v; br = r; }
         }
     }
+    /* the tiled kernel (variant 9, Horner form): columns per pass x tile
+     * height, on tiled forms built here on first use */
+    int bt = 0;
+    for (int ci = 0; ci < (int)(sizeof(tile_cand) / sizeof(tile_cand[0])) && !f; ci++) {
+        const int kc = tile_cand[ci][0], tr = tile_cand[ci][1];
+        if (kc > 8 && k <= kc / 2) continue;          /* more than half the lanes of a pass idle */
+        rsvdt_spmm_desc dt = dsc;
+        dt.lanes = kc;
+        dt.variant = 9;
+        if (!tile_attach(&dt, A, dir, tr)) continue;
+        for (int r = 0; r < 4; r++) {
+            dt.xcd_remap = r;
+            if (rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream)) continue;
+            rsvdt_hip_event_record(e0, g_rsvdt.stream);
+            for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream);
+            rsvdt_hip_event_record(e1, g_rsvdt.stream);
+            float ms = 0;
+            rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+            if (ms < best) { best = ms; bl = kc; bv = 9; br = r; bt = tr; }
+        }
+    }
     rsvdt_hip_event_destroy(e0);
     rsvdt_hip_event_destroy(e1);
     A->tuned_lanes[f][p][q] = bl;
     A->tuned_variant[f][p][q] = bv;
     A->tuned_remap[f][p][q] = br;
+    A->tuned_tile[f][p][q] = bt;
     A->tuned_ok[f][p][q] = 1;
     A->tuned_k[f][p][q] = k;
     if (g_rsvdt.opt.verbose)
-        rsvdt_printf("[rsvdt] autotune (%s, FP%d, %s): %d lanes/row, variant %d, xcd remap %d: %.1f us per stage\n",
-                     dir ? "A" : "A^H", prec, f ? "stage form" : "Horner form", bl, bv, br, 1e3 * best / 3);
+        rsvdt_printf("[rsvdt] autotune (%s, FP%d, %s): %d lanes/row, variant %d, xcd remap %d, tile %d rows: %.1f us "
+                     "per stage\n", dir ? "A" : "A^H", prec, f ? "stage form" : "Horner form", bl, bv, br, bt,
+                     1e3 * best / 3);
     return 0;
 }
 
@@ -446,67 +516,71 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     if (max_steps > 0 && max_steps < rend) rend = max_steps;
     stepper st;
     RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
+    /* From here on every failure is recorded in ierr and the loops stop, so
+     * that the tail (device sync, graph and event destruction) always runs:
+     * a forcing GEMM may still be writing a chunk buffer on the forcing
+     * stream when an error is met on the main stream. */
+    PetscErrorCode ierr = 0;
+#define TS_HIP(x)                                                                                      \
+    do {                                                                                               \
+        if (!ierr) {                                                                                   \
+            int _h = (x);                                                                              \
+            if (_h) ierr = rsvdt_err(_h, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error()); \
+        }                                                                                              \
+    } while (0)
     /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
     const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, prec);
     if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
-        RSVDT_CHK(autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2));
+        ierr = autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2);
         desc_for(&st.desc, A, dir, prec, k, 1);
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
+        TS_HIP(rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
     }
-    if (st.form == 0 && !fhat_folded) {        /* G_m = Q(z_m) F_hat_m in place of F_hat */
+    if (!ierr && st.form == 0 && !fhat_folded) {        /* G_m = Q(z_m) F_hat_m in place of F_hat */
         if (t_frc) pt_begin(t_frc);
-        RSVDT_CHK(folded_forcing(A, dir, V->TS.dt, k, Nwe, V->TS.Ns, D->idft->d, Fhat->d, Fhat->d));
+        ierr = folded_forcing(A, dir, V->TS.dt, k, Nwe, V->TS.Ns, D->idft->d, Fhat->d, Fhat->d);
         if (t_frc) pt_end(t_frc);
     }
     int64_t launches = 0;
     const PetscInt nch = (rend + st.S - 1) / st.S;
-    PetscErrorCode ierr = 0;
     /* forcing of the first two chunks; later chunks are refilled as soon as
      * the steps that read their buffer are done */
-    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_rk[1], g_rsvdt.stream));   /* zero fill done */
-    for (PetscInt c = 0; c < nch && c < 2; c++) {
+    TS_HIP(rsvdt_hip_event_record(st.ev_rk[1], g_rsvdt.stream));   /* zero fill done */
+    for (PetscInt c = 0; c < nch && c < 2 && !ierr; c++) {
         PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[1]));
+        TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[1]));
         if (t_frc) pt_begin_on(t_frc, frc_stream());
-        RSVDT_CHK(stepper_forcing(&st, (int)(c & 1), i0, steps));
+        if (!ierr) ierr = stepper_forcing(&st, (int)(c & 1), i0, steps);
         if (t_frc) pt_end_on(t_frc, frc_stream());
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
+        TS_HIP(rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
     }
     /* graph replay of full chunks (form 0): one graph per buffer */
     const int use_graph = g_rsvdt.opt.use_graph && st.form == 0 && !fused;
     void *gexec[2] = {NULL, NULL};
     int *slots = NULL;
-    if (use_graph) {
+    if (use_graph && !ierr) {
         slots = rsvdt_scratch(SCR_SLOTS, sizeof(int) * 2 * (size_t)st.S);
-        if (!slots) { stepper_free(&st); return rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate slot table"); }
+        if (!slots) ierr = rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate slot table");
     }
     for (PetscInt c = 0; c < nch && !ierr; c++) {
         const int buf = (int)(c & 1);
         const PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
+        TS_HIP(rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
+        if (ierr) break;
         if (t_rk4) pt_begin(t_rk4);
         if (fused) {
             const double cf[4] = {st.h / 4, st.h / 3, st.h / 2, st.h};
-            int hr = rsvdt_hip_horner_fused(&st.desc, A->nnz, dir ? A->maxrow : A->maxrowH, st.y, fslot(&st, buf, 0),
-                                            (int64_t)N * k, cf, steps,
-                                            Yall ? Yall->d : NULL, (int64_t)N * k, i0, V->TS.Nt, V->TS.ResRatio,
-                                            Nstore, dir, g_rsvdt.stream);
-            if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
+            TS_HIP(rsvdt_hip_horner_fused(&st.desc, A->nnz, dir ? A->maxrow : A->maxrowH, st.y, fslot(&st, buf, 0),
+                                          (int64_t)N * k, cf, steps, Yall ? Yall->d : NULL, (int64_t)N * k, i0,
+                                          V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream));
             launches += 1;
         } else if (use_graph && steps == st.S) {
             int *sl = slots + buf * st.S;
-            if (Yall) {
-                int hr = rsvdt_hip_snap_slots(sl, st.S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream);
-                if (hr) { ierr = rsvdt_err(hr, "TSActionRK4", "%s", rsvdt_hip_last_error()); break; }
-            }
-            if (!gexec[buf]) ierr = stepper_capture(&st, buf, sl, Yall ? Yall->d : NULL, &gexec[buf]);
-            if (!ierr) {
-                int hr = rsvdt_hip_graph_launch(gexec[buf], g_rsvdt.stream);
-                if (hr) ierr = rsvdt_err(hr, "TSActionRK4", "graph launch failed: %s", rsvdt_hip_last_error());
-            }
+            if (Yall) TS_HIP(rsvdt_hip_snap_slots(sl, st.S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream));
+            if (!ierr && !gexec[buf]) ierr = stepper_capture(&st, buf, sl, Yall ? Yall->d : NULL, &gexec[buf]);
+            TS_HIP(rsvdt_hip_graph_launch(gexec[buf], g_rsvdt.stream));
             launches += 4 * steps;
         }
-        for (PetscInt s = 0; s < steps && !fused && !(use_graph && steps == st.S); s++) {
+        for (PetscInt s = 0; s < steps && !ierr && !fused && !(use_graph && steps == st.S); s++) {
             PetscInt i = i0 + s;
             double *snap = NULL;
             if (Yall && i > V->TS.Nt && (i - V->TS.Nt) % V->TS.ResRatio == 0) {
@@ -516,22 +590,26 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
                 snap = Yall->d + 2 * (size_t)pos * (size_t)N * (size_t)k;
             }
             ierr = stepper_step(&st, buf, s, snap);
-            if (ierr) break;
             launches += 4;
         }
         if (t_rk4) pt_end(t_rk4);
-        RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_rk[buf], g_rsvdt.stream));
+        TS_HIP(rsvdt_hip_event_record(st.ev_rk[buf], g_rsvdt.stream));
         if (c + 2 < nch && !ierr) {
             const PetscInt j0 = 1 + (c + 2) * st.S, st2 = st.S < rend - j0 + 1 ? st.S : rend - j0 + 1;
-            RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[buf]));
+            TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[buf]));
             if (t_frc) pt_begin_on(t_frc, frc_stream());
-            ierr = stepper_forcing(&st, buf, j0, st2);
+            if (!ierr) ierr = stepper_forcing(&st, buf, j0, st2);
             if (t_frc) pt_end_on(t_frc, frc_stream());
-            RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_event_record(st.ev_frc[buf], frc_stream()));
+            TS_HIP(rsvdt_hip_event_record(st.ev_frc[buf], frc_stream()));
         }
     }
-    /* the main stream must not run ahead of a forcing GEMM still reading F_hat */
-    RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_sync());
+#undef TS_HIP
+    /* the main stream must not run ahead of a forcing GEMM still reading
+     * F_hat, and no buffer may be reused while one is writing: always wait */
+    {
+        int hs = rsvdt_hip_sync();
+        if (hs && !ierr) ierr = rsvdt_err(hs, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+    }
     for (int b = 0; b < 2; b++) rsvdt_hip_graph_destroy(gexec[b]);
     stepper_free(&st);
     /* in an action, chunk buffers of more than 1 GiB go back to the device:
@@ -613,45 +691,56 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
  * buffer like X of `bytes` bytes; the applies then alternate between X and
  * T (required for more than 64 columns with unit column strides, where the
  * apply cannot work in place) and the result is copied back into X. */
+PetscErrorCode rsvdt_cholqr_bufs(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
+                                 int64_t xa, double *R_out, int npass, int shift_first, double drop_tol, double *T,
+                                 size_t bytes, const rsvdt_cholqr_work *w, rsvdt_stream_t q, int report)
+{
+    const size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
+    RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_zero(w->info, 256, q));
+    const double shift_rel = 11.0 * ((double)nrows * (double)n + (double)n * (double)(n + 1)) * 1.1102230246251565e-16;
+    double *cur = X, *oth = T ? T : X;
+    for (int pass = 0; pass < npass; pass++) {
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, cur, xb, xr, xa, cur, xb, xr, xa, w->G, w->work, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, w->G, (pass == 0 && shift_first) ? shift_rel : 0.0,
+                                                       drop_tol, w->R, w->Ri, w->info, q));
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, cur, xb, xr, xa, w->Ri, 0.0, NULL, 0, 0, 0, oth, xb,
+                                                    xr, xa, q));
+        if (T) { double *tmp = cur; cur = oth; oth = tmp; }
+        if (R_out) {
+            if (pass == 0) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(w->Ra, w->R, mat, q));
+            else {
+                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_btrmm_upper(nb, n, w->R, w->Ra, w->Rt, q));
+                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(w->Ra, w->Rt, mat, q));
+            }
+        }
+    }
+    if (cur != X) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(X, cur, bytes, q));
+    if (R_out) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(R_out, w->Ra, mat, q));
+    if (report) {
+        int hinfo = 0;
+        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2h(&hinfo, w->info, sizeof(int)));
+        if (hinfo) rsvdt_printf("[rsvdt] warning: CholeskyQR met %d non-positive pivots (rank-deficient input)\n", hinfo);
+    }
+    return 0;
+}
+
 static PetscErrorCode cholqr_passes(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
                                     int64_t xa, double *R_out, int npass, int shift_first, double drop_tol,
                                     double *T, size_t bytes)
 {
     size_t mat = (size_t)nb * (size_t)n * (size_t)n * 16;
-    double *G = rsvdt_scratch(SCR_G, mat);
-    double *R = rsvdt_scratch(SCR_R, mat);
-    double *Ri = rsvdt_scratch(SCR_RINV, mat);
-    double *Ra = rsvdt_scratch(SCR_RACC, mat);
-    double *Rt = rsvdt_scratch(SCR_RTMP, mat);
-    void *work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(nb, nrows, n, n));
-    int *info = rsvdt_scratch(SCR_INFO, 256);
-    if (!G || !R || !Ri || !Ra || !Rt || !work || !info)
+    rsvdt_cholqr_work w;
+    w.G = rsvdt_scratch(SCR_G, mat);
+    w.R = rsvdt_scratch(SCR_R, mat);
+    w.Ri = rsvdt_scratch(SCR_RINV, mat);
+    w.Ra = R_out ? rsvdt_scratch(SCR_RACC, mat) : NULL;
+    w.Rt = R_out ? rsvdt_scratch(SCR_RTMP, mat) : NULL;
+    w.work = rsvdt_scratch(SCR_WORK, rsvdt_hip_bgram_work_bytes(nb, nrows, n, n));
+    w.info = rsvdt_scratch(SCR_INFO, 256);
+    if (!w.G || !w.R || !w.Ri || (R_out && (!w.Ra || !w.Rt)) || !w.work || !w.info)
         return rsvdt_err(RSVDT_ERR_MEM, "QRAllFreqs", "cannot allocate CholeskyQR work");
-    rsvdt_stream_t q = g_rsvdt.stream;
-    RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_zero(info, 256, q));
-    const double shift_rel = 11.0 * ((double)nrows * (double)n + (double)n * (double)(n + 1)) * 1.1102230246251565e-16;
-    double *cur = X, *oth = T ? T : X;
-    for (int pass = 0; pass < npass; pass++) {
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bgram(nb, nrows, n, n, cur, xb, xr, xa, cur, xb, xr, xa, G, work, q));
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bchol_inv(nb, n, G, (pass == 0 && shift_first) ? shift_rel : 0.0, drop_tol,
-                                                       R, Ri, info, q));
-        RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_bapply(nb, nrows, n, n, cur, xb, xr, xa, Ri, 0.0, NULL, 0, 0, 0, oth, xb, xr,
-                                                    xa, q));
-        if (T) { double *tmp = cur; cur = oth; oth = tmp; }
-        if (R_out) {
-            if (pass == 0) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, R, mat, q));
-            else {
-                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_btrmm_upper(nb, n, R, Ra, Rt, q));
-                RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(Ra, Rt, mat, q));
-            }
-        }
-    }
-    if (cur != X) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(X, cur, bytes, q));
-    if (R_out) RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2d(R_out, Ra, mat, q));
-    int hinfo = 0;
-    RSVDT_HIPCHK("QRAllFreqs", rsvdt_hip_memcpy_d2h(&hinfo, info, sizeof(int)));
-    if (hinfo) rsvdt_printf("[rsvdt] warning: CholeskyQR met %d non-positive pivots (rank-deficient input)\n", hinfo);
-    return 0;
+    return rsvdt_cholqr_bufs(nb, nrows, n, X, xb, xr, xa, R_out, npass, shift_first, drop_tol, T, bytes, &w,
+                             g_rsvdt.stream, 1);
 }
 
 PetscErrorCode rsvdt_cholqr(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr, int64_t xa,

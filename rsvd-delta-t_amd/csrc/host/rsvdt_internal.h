@@ -13,6 +13,21 @@
 /* ---- Mat --------------------------------------------------------------- */
 enum { RSVDT_MAT_DENSE = 1, RSVDT_MAT_AIJ = 2 };
 
+/* Tiled form of an operator for the LDS-staged stage kernel (rsvdt_spmm_desc
+ * variant 9): tiles of `rows` consecutive rows, each tile's distinct columns
+ * (d_tcol from d_tptr[t]) and each nonzero's position in its tile's list
+ * (d_lidx); hmax / emax: the most columns / nonzeros of a tile.  rows = 0:
+ * not built; rows < 0: not buildable (a tile with more than 65535 columns). */
+typedef struct {
+    int rows, hmax, emax;
+    int32_t *d_tptr, *d_tcol;
+    uint16_t *d_lidx;
+} rsvdt_tiled;
+/* tile heights the tiled stage kernel launches: (256 / columns per pass) x
+ * rows per slot */
+#define RSVDT_NTILE 5
+extern const int rsvdt_tile_rows[RSVDT_NTILE];
+
 /* Dense storage: the columns form nb = cols/kb blocks of kb columns; block b
  * is a rows x kb row-major slab at d + b*rows*kb (complex FP64).  order 0:
  * logical column c is (block c/kb, column c%kb) -- the frequency-major
@@ -45,10 +60,16 @@ struct _p_RSVDT_Mat {
     uint32_t *d_gword, *d_gwordH;
     void *d_gval, *d_gvalH;
     /* autotuned SpMM launch shape per [stepping form][precision 64/32]
-     * [direct/adjoint]: lanes per row, kernel variant, XCD remap; tuned_ok
-     * marks a tuned entry, valid for blocks of tuned_k columns */
+     * [direct/adjoint]: lanes per row, kernel variant, XCD remap, tile
+     * height (variant 9); tuned_ok marks a tuned entry, valid for blocks of
+     * tuned_k columns */
     int tuned_lanes[2][2][2], tuned_variant[2][2][2], tuned_remap[2][2][2], tuned_ok[2][2][2];
+    int tuned_tile[2][2][2];
     PetscInt tuned_k[2][2][2];   /* block width the entry was tuned for */
+    /* tiled forms [0: A, 1: A^H][tile height index], built on first use,
+     * from the host CSR structure of A and of A^H */
+    int64_t *h_rowptrH, *h_colH;
+    rsvdt_tiled tiled[2][RSVDT_NTILE];
 };
 
 struct _p_RSVDT_Vec {
@@ -101,6 +122,10 @@ PetscErrorCode rsvdt_mat_alloc_dense(PetscInt rows, PetscInt cols, PetscInt kb, 
 PetscErrorCode rsvdt_mat_dup(Mat src, Mat *out);
 PetscErrorCode rsvdt_mat_to_layout(Mat m, PetscInt kb, int order);   /* physical relayout if needed */
 PetscErrorCode rsvdt_aij_upload(Mat m);
+/* The tiled form of A (dir 1) or A^H (dir 0) with tiles of tile_rows rows,
+ * built and uploaded on first use; *out is NULL when it cannot be built
+ * (a tile with more than 65535 distinct columns, an unsupported height). */
+PetscErrorCode rsvdt_tiled_get(Mat m, int dir, int tile_rows, const rsvdt_tiled **out);
 PetscErrorCode rsvdt_aij_real_check(Mat m);
 
 /* ---- kernels of the action (rsvdt_action.c) ----------------------------- */
@@ -113,6 +138,14 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
  * drop_tol times the largest Gram diagonal are zeroed instead of normalized. */
 PetscErrorCode rsvdt_cholqr(PetscInt nbatch, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
                             int64_t xa, double *R_out, double drop_tol);
+/* The CholeskyQR passes on caller-provided device buffers (G, R, Ri:
+ * nbatch x n x n complex FP64; Ra, Rt only when R_out is wanted; work of
+ * rsvdt_hip_bgram_work_bytes bytes; info 256 bytes), enqueued on stream q.
+ * report: read the non-positive pivot count back (synchronises) and warn. */
+typedef struct { double *G, *R, *Ri, *Ra, *Rt; void *work; int *info; } rsvdt_cholqr_work;
+PetscErrorCode rsvdt_cholqr_bufs(PetscInt nb, PetscInt nrows, PetscInt n, double *X, int64_t xb, int64_t xr,
+                                 int64_t xa, double *R_out, int npass, int shift_first, double drop_tol, double *T,
+                                 size_t bytes, const rsvdt_cholqr_work *w, rsvdt_stream_t q, int report);
 /* Orthonormal basis Q of the column span of nbatch tall matrices X (same
  * strides; Q and the work arrays E and T each cover `bytes` bytes like X)
  * that may be numerically rank-deficient: CholeskyQR3 with dropping, then

@@ -448,6 +448,13 @@ PetscErrorCode RSVDT_MatDestroy(Mat *pm)
     rsvdt_hip_free(m->d_valz); rsvdt_hip_free(m->d_valzH);
     rsvdt_hip_free(m->d_gptr); rsvdt_hip_free(m->d_gword); rsvdt_hip_free(m->d_gval);
     rsvdt_hip_free(m->d_gptrH); rsvdt_hip_free(m->d_gwordH); rsvdt_hip_free(m->d_gvalH);
+    for (int d = 0; d < 2; d++)
+        for (int i = 0; i < RSVDT_NTILE; i++) {
+            rsvdt_hip_free(m->tiled[d][i].d_tptr);
+            rsvdt_hip_free(m->tiled[d][i].d_tcol);
+            rsvdt_hip_free(m->tiled[d][i].d_lidx);
+        }
+    free(m->h_rowptrH); free(m->h_colH);
     free(m->h_rowptr); free(m->h_col); free(m->h_val);
     free(m);
     *pm = NULL;
@@ -611,6 +618,94 @@ static PetscErrorCode upload_grouped(PetscInt rows, PetscInt cols, const int64_t
     return 0;
 }
 
+/* ---- tiled form (for the LDS-staged stage kernel) --------------------- */
+const int rsvdt_tile_rows[RSVDT_NTILE] = {8, 16, 32, 48, 64};
+
+typedef struct { int64_t col, e; } tent;
+static int tent_cmp(const void *a, const void *b)
+{
+    const tent *x = a, *y = b;
+    if (x->col != y->col) return x->col < y->col ? -1 : 1;
+    return x->e < y->e ? -1 : (x->e > y->e);
+}
+
+/* Tiles of TR consecutive rows: the sorted distinct columns of each tile and
+ * every nonzero's position among them. */
+static PetscErrorCode build_tiled(PetscInt rows, const int64_t *rp, const int64_t *ci, int TR, rsvdt_tiled *t)
+{
+    const int64_t nt = (rows + TR - 1) / TR, nnz = rp[rows];
+    if (nnz > 0x7fffffffLL || rows > 0x7fffffffLL) { t->rows = -1; return 0; }
+    int64_t emax = 0;
+    for (int64_t q = 0; q < nt; q++) {
+        const int64_t r0 = q * TR, r1 = r0 + TR < rows ? r0 + TR : rows;
+        if (rp[r1] - rp[r0] > emax) emax = rp[r1] - rp[r0];
+    }
+    int32_t *tptr = malloc(sizeof(int32_t) * (size_t)(nt + 1));
+    int32_t *tcol = malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    uint16_t *lidx = malloc(sizeof(uint16_t) * (size_t)(nnz ? nnz : 1));
+    tent *buf = malloc(sizeof(tent) * (size_t)(emax ? emax : 1));
+    if (!tptr || !tcol || !lidx || !buf) {
+        free(tptr); free(tcol); free(lidx); free(buf);
+        return rsvdt_err(RSVDT_ERR_MEM, "MatAIJ", "out of host memory (tiled form)");
+    }
+    int64_t h = 0, hmax = 0;
+    int ok = 1;
+    for (int64_t q = 0; q < nt && ok; q++) {
+        const int64_t r0 = q * TR, r1 = r0 + TR < rows ? r0 + TR : rows;
+        const int64_t e0 = rp[r0], n = rp[r1] - e0;
+        for (int64_t i = 0; i < n; i++) buf[i] = (tent){ci[e0 + i], e0 + i};
+        qsort(buf, (size_t)n, sizeof(tent), tent_cmp);
+        tptr[q] = (int32_t)h;
+        for (int64_t i = 0; i < n; i++) {
+            if (i == 0 || buf[i].col != buf[i - 1].col) tcol[h++] = (int32_t)buf[i].col;
+            const int64_t slot = h - 1 - tptr[q];
+            if (slot > 65535) { ok = 0; break; }
+            lidx[buf[i].e] = (uint16_t)slot;
+        }
+        if (h - tptr[q] > hmax) hmax = h - tptr[q];
+    }
+    tptr[nt] = (int32_t)h;
+    free(buf);
+    if (!ok) {
+        free(tptr); free(tcol); free(lidx);
+        t->rows = -1;
+        return 0;
+    }
+    int rc = rsvdt_hip_malloc((void **)&t->d_tptr, sizeof(int32_t) * (size_t)(nt + 1));
+    if (!rc) rc = rsvdt_hip_malloc((void **)&t->d_tcol, sizeof(int32_t) * (size_t)(h ? h : 1));
+    if (!rc) rc = rsvdt_hip_malloc((void **)&t->d_lidx, sizeof(uint16_t) * (size_t)(nnz ? nnz : 1));
+    if (!rc) rc = rsvdt_hip_memcpy_h2d(t->d_tptr, tptr, sizeof(int32_t) * (size_t)(nt + 1));
+    if (!rc && h) rc = rsvdt_hip_memcpy_h2d(t->d_tcol, tcol, sizeof(int32_t) * (size_t)h);
+    if (!rc && nnz) rc = rsvdt_hip_memcpy_h2d(t->d_lidx, lidx, sizeof(uint16_t) * (size_t)nnz);
+    free(tptr); free(tcol); free(lidx);
+    if (rc) {
+        rsvdt_hip_free(t->d_tptr); rsvdt_hip_free(t->d_tcol); rsvdt_hip_free(t->d_lidx);
+        t->d_tptr = t->d_tcol = NULL; t->d_lidx = NULL;
+        return rsvdt_err(rc, "MatAIJ", "%s", rsvdt_hip_last_error());
+    }
+    t->rows = TR;
+    t->hmax = (int)hmax;
+    t->emax = (int)emax;
+    return 0;
+}
+
+PetscErrorCode rsvdt_tiled_get(Mat m, int dir, int tile_rows, const rsvdt_tiled **out)
+{
+    *out = NULL;
+    RSVDT_CHK(rsvdt_aij_upload(m));
+    int ix = -1;
+    for (int i = 0; i < RSVDT_NTILE; i++) if (rsvdt_tile_rows[i] == tile_rows) ix = i;
+    if (ix < 0) return 0;
+    const int d = dir ? 0 : 1;
+    rsvdt_tiled *t = &m->tiled[d][ix];
+    if (t->rows == 0) {
+        if (dir) RSVDT_CHK(build_tiled(m->rows, m->h_rowptr, m->h_col, tile_rows, t));
+        else RSVDT_CHK(build_tiled(m->cols, m->h_rowptrH, m->h_colH, tile_rows, t));
+    }
+    if (t->rows > 0) *out = t;
+    return 0;
+}
+
 /* Upload A and A^H (built on the host by a counting transpose).  Called
  * lazily on first device use, so that loading and validating matrices needs
  * no GPU. */
@@ -650,7 +745,10 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     m->maxrowH = 0;
     for (PetscInt r = 0; r < nr; r++)
         if (rp[r + 1] - rp[r] > m->maxrowH) m->maxrowH = rp[r + 1] - rp[r];
-    free(rp); free(ci); free(v); free(fill);
+    free(m->h_rowptrH); free(m->h_colH);
+    m->h_rowptrH = rp;      /* kept for the tiled form of A^H */
+    m->h_colH = ci;
+    free(v); free(fill);
     memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
     return ierr;
 }

@@ -403,12 +403,15 @@ FAST32 = dict(precision=32, rk4_form=0, fuse_small=1)
 
 
 def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242, fast=FAST, plain=PLAIN,
-                  extra=(("fast_fp32", FAST32),)):
+                  extra=(("fast_fp32", FAST32, True), ("fast_random", FAST, False))):
     """Wall time to sigma_1 relative error <= tol (against a converged
     reference: q_ref power iterations, optimised form) for the plain form
     with a random sketch and for the optimised form with the learned sketch,
-    each with the fewest power iterations that reach tol.  Returns per-member
-    rows and the mean times."""
+    each with the fewest power iterations that reach tol; `extra` adds forms
+    (name, options, learned columns or not): by default the optimised form
+    with the FP32 state, and the optimised FP64 form with a random sketch --
+    the engine's best form without the network, which isolates what the
+    learned columns buy.  Returns per-member rows and the mean times."""
     N = FAMILY["N"]
     x = np.linspace(-85, 85, N)
     rows = []
@@ -417,7 +420,7 @@ def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242,
         Sref, _, _ = rsvd_run(pkg, A, k, q_ref, seed + 7919 * i, **fast)
         om = omegas(FAMILY["Nw"], FAMILY["w"], False)[:Sref.shape[1]]
         row = {"mu0": mu0, "Ur": Ur}
-        for name, opts, lrn in (("plain", plain, False), ("fast", fast, True)) + tuple((n, o, True) for n, o in extra):
+        for name, opts, lrn in (("plain", plain, False), ("fast", fast, True)) + tuple(extra):
             for q in range(qmax + 1):
                 t0 = time.time()
                 cols = net.columns(mu0, Ur, x, om) if lrn else None
@@ -433,9 +436,9 @@ def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242,
     tf = float(np.mean([r["fast"]["wall_s"] for r in rows]))
     summ = {"plain_s": tp, "fast_s": tf, "speedup": tp / tf, "all_reach_tol": ok,
             "plain_q": [r["plain"]["q"] for r in rows], "fast_q": [r["fast"]["q"] for r in rows]}
-    for n, _ in extra:
+    for n, _, _ in extra:
         te = float(np.mean([r[n]["wall_s"] for r in rows]))
-        summ[n] = {"s": te, "speedup": tp / te, "q": [r[n]["q"] for r in rows],
+        summ[n] = {"s": te, "speedup": tp / te, "speedup_of_fast": te / tf, "q": [r[n]["q"] for r in rows],
                    "max_sigma1_err": float(max(r[n]["sigma1_err"] for r in rows)),
                    "all_reach_tol": all(r[n]["sigma1_err"] <= tol for r in rows)}
     return rows, summ

@@ -47,7 +47,33 @@ class Desc(ctypes.Structure):
     """Mirror of rsvdt_spmm_desc (include/rsvdt_hip.h)."""
     _fields_ = [("nrows", I64_), ("k", I64_), ("nb", I64_), ("rowptr", V_), ("col", V_), ("val", V_),
                 ("a_real", ctypes.c_int), ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int),
-                ("variant", ctypes.c_int), ("group", ctypes.c_int), ("gptr", V_), ("gword", V_), ("gval", V_)]
+                ("variant", ctypes.c_int), ("group", ctypes.c_int), ("gptr", V_), ("gword", V_), ("gval", V_),
+                ("tile_rows", ctypes.c_int), ("tile_hmax", ctypes.c_int), ("tile_emax", ctypes.c_int),
+                ("tptr", V_), ("tcol", V_), ("lidx", V_)]
+
+
+def tiled_form(A, TR):
+    """Tiled form of a CSR matrix (rsvdt_spmm_desc.tile_rows/tptr/tcol/lidx):
+    per tile of TR consecutive rows its distinct columns, ascending, and for
+    each nonzero the position of its column in its tile's list -- built here
+    with numpy, independently of the library's own builder.  Returns
+    (tptr, tcol, lidx, hmax, emax)."""
+    A = A.tocsr()
+    A.sort_indices()
+    N = A.shape[0]
+    nt = (N + TR - 1) // TR
+    rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(A.indptr))
+    tile = rows // TR
+    key = tile * N + A.indices.astype(np.int64)
+    uniq, inv = np.unique(key, return_inverse=True)
+    counts = np.bincount(uniq // N, minlength=nt)
+    tptr = np.zeros(nt + 1, np.int32)
+    np.cumsum(counts, out=tptr[1:])
+    lidx = (inv - tptr[tile]).astype(np.uint16)
+    tcol = (uniq % N).astype(np.int32)
+    ends = np.minimum(np.arange(1, nt + 1) * TR, N)
+    emax = int(np.max(A.indptr[ends] - A.indptr[np.arange(nt) * TR]))
+    return tptr, tcol, lidx, int(counts.max()), emax
 
 
 def grouped_form(A, G):

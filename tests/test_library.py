@@ -185,3 +185,23 @@ def test_setup_grid_matches(pkg, w, Nw, TL, dt):
     TR = pkg.TransRun_vars()
     assert pkg.lib().SetupTimeFreqGrid(ctypes.byref(V), ctypes.byref(TR)) == 0
     assert (V.TS.dt, V.TS.Ns, V.TS.Nt, V.TS.ResRatio) == kat.grid(w, Nw, TL, dt)
+
+
+def test_matview_reproduces_reference_file(pkg, tmp_path):
+    """MatLoad then MatView of the reference's own operator file writes its
+    first object (the Mat: header, row lengths, columns, values -- explicit
+    zeros included) byte for byte; the trailing Vec object of the file is not
+    part of the Mat (SourceCode/LNSStructure.c:26 reads only the Mat)."""
+    L = pkg.lib()
+    m = ctypes.c_void_p()
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "A_GL")
+    assert L.RSVDT_MatLoad(src.encode(), 1, ctypes.byref(m)) == 0
+    out = str(tmp_path / "A_view")
+    assert L.RSVDT_MatView(m, out.encode()) == 0
+    pkg.destroy(m)
+    ref = open(src, "rb").read()
+    hdr = np.frombuffer(ref[:32], dtype=">i8")
+    M, nz = int(hdr[1]), int(hdr[3])
+    nbytes = 32 + 8 * M + 8 * nz + 16 * nz
+    got = open(out, "rb").read()
+    assert len(got) == nbytes and got == ref[:nbytes]
