@@ -53,6 +53,11 @@ int rsvdt_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
 int rsvdt_hip_stream_create(rsvdt_stream_t *s);
 int rsvdt_hip_stream_destroy(rsvdt_stream_t s);
 int rsvdt_hip_stream_wait_event(rsvdt_stream_t s, void *ev);
+int rsvdt_hip_stream_sync(rsvdt_stream_t s);
+/* pinned host memory, and asynchronous device-to-host copies into it */
+int rsvdt_hip_host_alloc(void **p, size_t bytes);
+int rsvdt_hip_host_free(void *p);
+int rsvdt_hip_memcpy_d2h_async(void *dst_host, const void *src, size_t bytes, rsvdt_stream_t s);
 
 /* events for timing on a given stream */
 int rsvdt_hip_event_create(void **ev);

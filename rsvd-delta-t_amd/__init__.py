@@ -117,7 +117,8 @@ ENTRY_POINTS = [
 HIP_ENTRY_POINTS = [
     "rsvdt_hip_init", "rsvdt_hip_last_error", "rsvdt_hip_malloc", "rsvdt_hip_free", "rsvdt_hip_memcpy_h2d",
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
-    "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event",
+    "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event", "rsvdt_hip_stream_sync",
+    "rsvdt_hip_host_alloc", "rsvdt_hip_host_free", "rsvdt_hip_memcpy_d2h_async",
     "rsvdt_hip_event_create", "rsvdt_hip_event_destroy", "rsvdt_hip_event_record", "rsvdt_hip_event_elapsed_ms",
     "rsvdt_hip_rk4_stage", "rsvdt_hip_horner_stage", "rsvdt_hip_horner_fused", "rsvdt_hip_horner_fused_fits", "rsvdt_hip_horner_stage_ex", "rsvdt_hip_snap_slots",
     "rsvdt_hip_graph_begin", "rsvdt_hip_graph_end", "rsvdt_hip_graph_launch", "rsvdt_hip_graph_destroy", "rsvdt_hip_spmm_blocks", "rsvdt_hip_coef_gemm", "rsvdt_hip_bgram_work_bytes",

@@ -95,6 +95,98 @@ __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restr
     if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
 }
 
+// The same with the factor and its inverse held in LDS for n <= 70 (2*n*n*16
+// bytes of dynamic LDS): every read of the elimination and of the triangular inverse stays on
+// the CU; the global arrays are read once and written once.  The inverse is
+// formed row by row from the bottom, all columns of a row in parallel:
+// X(i, c) = -(sum_{l=i+1..c} R(i,l) X(l,c)) / R(i,i), X(c, c) = 1 / R(c,c).
+__global__ __launch_bounds__(256) void k_bchol_inv_lds(int n, const double2 *__restrict__ G, double shift_rel,
+                                                       double drop_tol, double2 *__restrict__ R,
+                                                       double2 *__restrict__ Rinv, int *info)
+{
+    extern __shared__ double2 cl[];          // R then X, n*n each
+    double2 *sR = cl, *sX = cl + n * n;
+    __shared__ double s_piv, s_shift, s_dmax;
+    __shared__ int s_bad;
+    const int64_t b = blockIdx.x;
+    const double2 *Gb = G + b * n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        sR[e] = (i <= c) ? Gb[e] : make_double2(0, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {                  // trace and largest diagonal entry: one wave
+        double tr = 0, dm = 0;
+        for (int i = threadIdx.x; i < n; i += 64) { tr += sR[i * n + i].x; dm = fmax(dm, sR[i * n + i].x); }
+        for (int o = 32; o > 0; o >>= 1) {
+            tr += __shfl_xor(tr, o, 64);
+            dm = fmax(dm, __shfl_xor(dm, o, 64));
+        }
+        if (threadIdx.x == 0) { s_shift = shift_rel * tr; s_dmax = dm; s_bad = 0; }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double2 v = sR[i * n + i];
+        v.x += s_shift;
+        v.y = 0;
+        sR[i * n + i] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (threadIdx.x == 0) {
+            double d = sR[j * n + j].x;
+            if (drop_tol > 0 && !(d > drop_tol * s_dmax)) {
+                s_piv = 0;                      // column j lies in the span of columns < j: drop it
+            } else {
+                if (!(d > 1e-300)) { d = 1e-300; s_bad++; }
+                s_piv = sqrt(d);
+            }
+            sR[j * n + j] = make_double2(s_piv, 0);
+        }
+        __syncthreads();
+        const double inv = s_piv > 0 ? 1.0 / s_piv : 0.0;
+        for (int c = j + 1 + threadIdx.x; c < n; c += blockDim.x) {
+            const double2 v = sR[j * n + c];
+            sR[j * n + c] = make_double2(v.x * inv, v.y * inv);
+        }
+        __syncthreads();
+        const int m = n - j - 1;
+        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+            const int i = j + 1 + e / m, c = j + 1 + e % m;
+            if (i <= c) {
+                const double2 ri = sR[j * n + i], rc = sR[j * n + c];
+                double2 v = sR[i * n + c];
+                v.x -= ri.x * rc.x + ri.y * rc.y;
+                v.y -= ri.x * rc.y - ri.y * rc.x;
+                sR[i * n + c] = v;
+            }
+        }
+        __syncthreads();
+    }
+    // inverse, bottom row first; dropped columns (zero pivot) give zero rows and columns
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) sX[e] = make_double2(0, 0);
+    __syncthreads();
+    for (int i = n - 1; i >= 0; --i) {
+        const double rii = sR[i * n + i].x;
+        for (int c = i + threadIdx.x; c < n; c += blockDim.x) {
+            double2 v = make_double2(0, 0);
+            if (rii > 0) {
+                if (c == i) v = make_double2(1.0 / rii, 0);
+                else if (sR[c * n + c].x > 0) {
+                    double2 acc = make_double2(0, 0);
+                    for (int l = i + 1; l <= c; ++l) acc = zadd(acc, zmul(sR[i * n + l], sX[l * n + c]));
+                    v = make_double2(-acc.x / rii, -acc.y / rii);
+                }
+            }
+            sX[i * n + c] = v;
+        }
+        __syncthreads();
+    }
+    double2 *Rb = R + b * n * n, *Xb = Rinv + b * n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) { Rb[e] = sR[e]; Xb[e] = sX[e]; }
+    if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
+}
+
 // D_b = diag(1 where R_b(j,j) is not positive, else 0): the columns that a
 // dropping CholeskyQR zeroed (their R has a zero pivot); count += their number.
 __global__ void k_bdrop_mask(int64_t tot, int n, const double2 *__restrict__ R, double2 *__restrict__ D, int *count)
@@ -324,6 +416,91 @@ __global__ __launch_bounds__(256) void k_bshift_solve(int n, int nrhs, const dou
     if (threadIdx.x == 0 && bad && info != nullptr) atomicAdd(info, 1);
 }
 
+// The same with W and the right-hand side in LDS for n <= 96: the pivot
+// search is one wave's shuffle reduction, and each elimination step is two
+// barriers.  Row-major W(i, c) = shift*I - M_b.
+__global__ __launch_bounds__(256) void k_bshift_solve_lds(int n, int nrhs, const double2 *__restrict__ M,
+                                                          const double2 *__restrict__ shift, const double2 *B,
+                                                          double2 *X, int *info)
+{
+    extern __shared__ double2 sl[];          // W (n*n), then x (n)
+    double2 *W = sl, *x = sl + n * n;
+    __shared__ int s_p;
+    __shared__ int s_bad;
+    const int64_t sys = blockIdx.x;
+    const int64_t b = sys / nrhs;
+    const int m = (int)(sys % nrhs);
+    const double2 *Mb = M + b * n * n;
+    const double2 sh = shift[b * nrhs + m];
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, c = e % n;
+        const double2 v = Mb[e];
+        double2 w = make_double2(-v.x, -v.y);
+        if (i == c) { w.x += sh.x; w.y += sh.y; }
+        W[e] = w;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = B[(b * n + i) * nrhs + m];
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (threadIdx.x < 64) {              // pivot: largest |W(i, j)|, i >= j (lowest index on ties)
+            double best = -1;
+            int bi = j;
+            for (int i = j + (int)threadIdx.x; i < n; i += 64) {
+                const double a = zabs2(W[i * n + j]);
+                if (a > best) { best = a; bi = i; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(best, o, 64);
+                const int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+            }
+            if (threadIdx.x == 0) {
+                s_p = bi;
+                if (!(best > 0)) s_bad = 1;
+            }
+        }
+        __syncthreads();
+        const int p = s_p;
+        if (p != j) {                        // swap rows j and p (columns j.. and the rhs)
+            for (int c = j + threadIdx.x; c <= n; c += blockDim.x) {
+                double2 *a = c < n ? &W[j * n + c] : &x[j];
+                double2 *q = c < n ? &W[p * n + c] : &x[p];
+                const double2 t = *a;
+                *a = *q;
+                *q = t;
+            }
+            __syncthreads();
+        }
+        const double2 d = W[j * n + j];
+        const double dd = zabs2(d);
+        const double2 dinv = dd > 0 ? make_double2(d.x / dd, -d.y / dd) : make_double2(0, 0);
+        // rows i > j: l = W(i,j)/d, W(i, c) -= l W(j, c) for c > j, x(i) -= l x(j);
+        // one thread per (row, column) pair, the multiplier recomputed per pair
+        const int mm = n - j - 1;
+        for (int e = threadIdx.x; e < mm * (mm + 1); e += blockDim.x) {
+            const int i = j + 1 + e / (mm + 1), c = j + 1 + e % (mm + 1);
+            const double2 l = zmul(W[i * n + j], dinv);
+            if (c < n) W[i * n + c] = zsub(W[i * n + c], zmul(l, W[j * n + c]));
+            else x[i] = zsub(x[i], zmul(l, x[j]));
+        }
+        __syncthreads();
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        if (threadIdx.x == 0) {
+            const double2 d = W[c * n + c];
+            const double dd = zabs2(d);
+            x[c] = dd > 0 ? zmul(x[c], make_double2(d.x / dd, -d.y / dd)) : make_double2(0, 0);
+        }
+        __syncthreads();
+        const double2 xc = x[c];
+        for (int i = threadIdx.x; i < c; i += blockDim.x) x[i] = zsub(x[i], zmul(W[i * n + c], xc));
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[(b * n + i) * nrhs + m] = x[i];
+    if (threadIdx.x == 0 && s_bad && info != nullptr) atomicAdd(info, 1);
+}
+
 }  // namespace rsvdt
 
 using namespace rsvdt;
@@ -333,8 +510,12 @@ extern "C" int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, d
 {
     if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
     if (n > 1024 || nbatch > 0x7fffffff) { set_error_msg("rsvdt_hip_bchol_inv: n <= 1024"); return RSVDT_HIP_ERR_ARG; }
-    hipLaunchKernelGGL(k_bchol_inv, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n, (const double2 *)G,
-                       shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
+    if (n <= 70)      // R and R^-1 in LDS (2 n^2 x 16 B <= 154 KB)
+        hipLaunchKernelGGL(k_bchol_inv_lds, dim3((unsigned)nbatch), dim3(256), (size_t)2 * n * n * 16, as_stream(s),
+                           (int)n, (const double2 *)G, shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
+    else
+        hipLaunchKernelGGL(k_bchol_inv, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n,
+                           (const double2 *)G, shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_bchol_inv");
     return RSVDT_HIP_OK;
 }
@@ -381,9 +562,14 @@ extern "C" int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs, c
 {
     if (nbatch <= 0 || n <= 0 || nrhs <= 0) return RSVDT_HIP_OK;
     if (n > 4096 || nbatch * nrhs > 0x7fffffffLL) { set_error_msg("rsvdt_hip_bshift_solve: bad size"); return RSVDT_HIP_ERR_ARG; }
-    hipLaunchKernelGGL(k_bshift_solve, dim3((unsigned)(nbatch * nrhs)), dim3(256), (size_t)n * 16, as_stream(s),
-                       (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift, (const double2 *)B,
-                       (double2 *)X, (double2 *)work, info);
+    if (n <= 96)      // W and the right-hand side in LDS ((n^2 + n) x 16 B <= 149 KB)
+        hipLaunchKernelGGL(k_bshift_solve_lds, dim3((unsigned)(nbatch * nrhs)), dim3(256), (size_t)(n * n + n) * 16,
+                           as_stream(s), (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift,
+                           (const double2 *)B, (double2 *)X, info);
+    else
+        hipLaunchKernelGGL(k_bshift_solve, dim3((unsigned)(nbatch * nrhs)), dim3(256), (size_t)n * 16, as_stream(s),
+                           (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift, (const double2 *)B,
+                           (double2 *)X, (double2 *)work, info);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_bshift_solve");
     return RSVDT_HIP_OK;
 }

@@ -83,6 +83,25 @@ extern "C" int rsvdt_hip_stream_destroy(rsvdt_stream_t s) {
     if (s) RSVDT_HIP_TRY(hipStreamDestroy((hipStream_t)s));
     return RSVDT_HIP_OK;
 }
+extern "C" int rsvdt_hip_stream_sync(rsvdt_stream_t s) {
+    RSVDT_HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+    return RSVDT_HIP_OK;
+}
+// Pinned host memory (hipHostMalloc) for streamed device-to-host transfers.
+extern "C" int rsvdt_hip_host_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    RSVDT_HIP_TRY(hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault));
+    return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_host_free(void *p) {
+    if (p) RSVDT_HIP_TRY(hipHostFree(p));
+    return RSVDT_HIP_OK;
+}
+// Asynchronous copy into pinned host memory, ordered on stream s.
+extern "C" int rsvdt_hip_memcpy_d2h_async(void *dst_host, const void *src, size_t bytes, rsvdt_stream_t s) {
+    if (bytes) RSVDT_HIP_TRY(hipMemcpyAsync(dst_host, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
+    return RSVDT_HIP_OK;
+}
 extern "C" int rsvdt_hip_stream_wait_event(rsvdt_stream_t s, void *ev) {
     RSVDT_HIP_TRY(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)ev, 0));
     return RSVDT_HIP_OK;
