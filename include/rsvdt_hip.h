@@ -241,6 +241,22 @@ int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs,
                            const double *B, double *X, void *work, int *info,
                            rsvdt_stream_t s);
 
+/* ---- per-frequency QR without device allocation ----------------------- *
+ * No counterpart in the reference: the in-place QR of QRAllFreqs
+ * (shifted CholeskyQR3 of each frequency block, the same kernels and the
+ * same result) for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Y holds nblocks N x k row-major blocks (block b at b*N*k complex entries),
+ * k <= 64; the call is asynchronous on s and reads nothing back (a block
+ * without full column rank is not reported).  Its work lives in one static
+ * device array of RSVDT_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; sizes whose work (rsvdt_qr_blocks_noalloc_bytes) exceeds it are
+ * rejected with RSVDT_HIP_ERR_ARG.  Calls share that array: calls that
+ * overlap on one GPU (unordered streams, several host threads) race.     */
+#define RSVDT_NOALLOC_SCRATCH_BYTES ((size_t)192 << 20)   /* covers k <= 64 at any N, ~300 blocks of k = 64 */
+size_t rsvdt_qr_blocks_noalloc_bytes(int64_t nblocks, int64_t N, int64_t k);
+int rsvdt_qr_blocks_noalloc(double *Y, int64_t N, int64_t k, int64_t nblocks, rsvdt_stream_t s);
+
 /* ---- elementwise ------------------------------------------------------ */
 /* y = a*x (complex entries, FP64), n entries */
 int rsvdt_hip_zscale(int64_t n, double a, const double *x, double *y, rsvdt_stream_t s);

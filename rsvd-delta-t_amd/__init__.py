@@ -126,6 +126,7 @@ HIP_ENTRY_POINTS = [
     "rsvdt_hip_bjacobi_svd", "rsvdt_hip_bshift_solve_work_bytes", "rsvdt_hip_bshift_solve", "rsvdt_hip_zscale", "rsvdt_hip_zaxpy",
     "rsvdt_hip_z2c", "rsvdt_hip_c2z", "rsvdt_hip_zcopy3", "rsvdt_hip_transpose_rows", "rsvdt_hip_fill_random", "rsvdt_hip_znorm2", "rsvdt_hip_mlp_features", "rsvdt_hip_gemm_bf16",
     "rsvdt_hip_colsum", "rsvdt_hip_sketch_loss", "rsvdt_hip_adam", "rsvdt_hip_trans_synth",
+    "rsvdt_qr_blocks_noalloc_bytes", "rsvdt_qr_blocks_noalloc",
 ]
 
 _lib = None

@@ -16,6 +16,8 @@ using namespace rsvdt;
 
 extern "C" const char *rsvdt_hip_last_error(void) { return g_err; }
 
+extern "C" int rsvdt_noalloc_setup(int device);
+
 extern "C" int rsvdt_hip_init(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -36,7 +38,7 @@ extern "C" int rsvdt_hip_init(int device) {
         set_error_msg(msg);
         return RSVDT_HIP_ERR_NODEV;
     }
-    return RSVDT_HIP_OK;
+    return rsvdt_noalloc_setup(device);     // device address of the no-allocation entry point's scratch
 }
 
 extern "C" int rsvdt_hip_malloc(void **p, size_t bytes) {
