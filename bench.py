@@ -346,6 +346,98 @@ def calib_copy(pkg, gib=1, reps=5):
                            "kernel": "k_zcopy3"}}
 
 
+def subspace_gap(U1, U2):
+    """sin of the largest principal angle between span(U1) and span(U2)
+    (orthonormal columns)."""
+    return float(np.linalg.norm(U2 - U1 @ (U1.conj().T @ U2), 2))
+
+
+def fp32_form_c2(pkg, TLT=6.0, nmodes=3):
+    """The precision form on the 2D flow operator (N = 393,216, k = 16,
+    Nw = 128 -> 64 frequencies, q = 1, transient removal, TL = 6 T -- the
+    same setting for both forms): the whole main.c sequence with the RK4 state
+    and SpMM in complex FP32 (DFT, CholeskyQR, SVD and the removal in FP64)
+    against the FP64 form, on the same seeded sketch.  Per frequency: the
+    largest relative error of sigma_1..sigma_nmodes and the gap between the
+    leading nmodes response (U) and forcing (V) subspaces.  At TL = 6T the
+    FP64 form is converged (the removal's correction of the ~e^{-6} residual
+    transient is small), and so is the FP32 form; at TL = T/2 the removal's
+    correction of the large transient carries the FP32 rounding into the
+    gains at 1e-5..1e-4 (reduced-grid study)."""
+    A, k, Nw, w, dt, desc = workload("flow2d")
+    T = 2 * np.pi / w
+    res = {}
+    for prec in (64, 32):
+        s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=TLT * T, dt=dt, transient_removal=True, precision=prec)
+        s.random_sketch(seed=1373, gaussian=True)
+        pkg.lib().rsvdt_hip_sync()
+        t0 = time.time()
+        for fn in (s.direct, s.power_iteration, s.svd_before_adjoint, s.store_u, s.adjoint, s.svd_allfreqs):
+            fn()
+        pkg.lib().rsvdt_hip_sync()
+        wall = time.time() - t0
+        U = pkg.mat_to_dense(pkg._vp(s.Res.U_hat))
+        Vh = pkg.mat_to_dense(pkg._vp(s.Res.V_hat))
+        S = pkg.mat_to_dense(pkg._vp(s.Res.S_hat)).real
+        nwe = s.Nw_eff
+        s.close()
+        pkg.lib().RSVDT_Finalize()
+        res[prec] = (U, Vh, S, wall)
+        log(f"C2 FP{prec}: {wall:.1f} s")
+    U64, V64, S64, t64 = res[64]
+    U32, V32, S32, t32 = res[32]
+    es = np.abs(S32[:nmodes] - S64[:nmodes]) / S64[:nmodes]
+    gu = [subspace_gap(U64[:, m * k:m * k + nmodes], U32[:, m * k:m * k + nmodes]) for m in range(nwe)]
+    gv = [subspace_gap(V64[:, m * k:m * k + nmodes], V32[:, m * k:m * k + nmodes]) for m in range(nwe)]
+    return {"workload": f"{desc}, k={k}, Nw={Nw} (Nw_eff={nwe}), q=1, transient removal, TL={TLT:g}T (both forms)",
+            "fp64_s": round(t64, 2), "fp32_s": round(t32, 2), "time_ratio_fp64_over_fp32": round(t64 / t32, 3),
+            "sigma_rel_err_max": float(es.max()), "sigma1_rel_err_max": float(es[0].max()),
+            "sigma1_rel_err_median": float(np.median(es[0])), "within_1e-6": bool(es[0].max() <= 1e-6),
+            f"sigma1_{nmodes}_rel_err_per_freq": [float(x) for x in es.max(axis=0)],
+            f"U_top{nmodes}_subspace_gap_per_freq": [float(x) for x in gu],
+            f"V_top{nmodes}_subspace_gap_per_freq": [float(x) for x in gv],
+            "U_gap_max": float(max(gu)), "V_gap_max": float(max(gv))}
+
+
+def save_results_c3(pkg, folder, N=1966080, k=32, nwe=32, opt=2):
+    """SaveResults (SourceCode/SaveResults.c:32-92) of results of the 3D
+    workload's size -- U_hat and V_hat of N x k*Nw_eff complex FP64 (32 GB
+    each) resident on the device -- into `folder`, timed end to end (file
+    bytes / wall time), the files removed afterwards."""
+    import shutil
+    L = pkg.lib()
+    if L.RSVDT_Initialize(None, None) != 0:
+        raise RuntimeError("RSVDT_Initialize failed")
+    res = pkg.Resolvent_matrices()
+    handles = []
+    for name in ("U_hat", "V_hat"):
+        m = ctypes.c_void_p()
+        pkg.check(L.RSVDT_MatCreateDenseBlocked(N, k * nwe, k, None, ctypes.byref(m)), "RSVDT_MatCreateDenseBlocked")
+        setattr(res, name, m.value)
+        handles.append(m.value)
+    res.S_hat = pkg.mat_from_dense(np.ones((k, nwe), np.complex128)).value
+    handles.append(res.S_hat)
+    V = pkg.RSVDt_vars()
+    V.RSVD.k, V.RSVD.Nw_eff, V.SaveResultsOpt = k, nwe, opt
+    d = pkg.Directories()
+    os.makedirs(folder, exist_ok=True)
+    d.FolderDir = (os.path.abspath(folder) + "/").encode()
+    pkg.lib().rsvdt_hip_sync()
+    t0 = time.time()
+    rc = L.SaveResults(ctypes.byref(res), ctypes.byref(V), ctypes.byref(d))
+    wall = time.time() - t0
+    nbytes = sum(os.path.getsize(os.path.join(folder, f)) for f in os.listdir(folder))
+    nfiles = len(os.listdir(folder))
+    shutil.rmtree(folder, ignore_errors=True)
+    for h in handles:
+        pkg.destroy(h)
+    pkg.check(rc, "SaveResults")
+    return {"opt": opt, "files": nfiles, "file_bytes": nbytes, "seconds": round(wall, 2),
+            "GBps": round(nbytes / wall / 1e9, 2), "folder": folder,
+            "data": f"U_hat, V_hat: {N} x {k * nwe} complex FP64 each ({N * k * nwe * 16 / 1e9:.1f} GB), "
+                    "on the device"}
+
+
 def time_to_modes_gl(pkg, nmembers):
     """Time to sigma_1 relative error <= 1e-6 on held-out members of the
     Ginzburg-Landau family (N=500, k=16, Nw=64, transient removal): the
@@ -386,9 +478,15 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline workload (no flow2d, FP32, MFMA, streaming, time-to-modes keys)")
     ap.add_argument("--no-ttm", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the FP32-form measurement on the 2D operator")
     ap.add_argument("--ttm-members", type=int, default=50)
-    ap.add_argument("--only", choices=("mfma", "stream"), default=None,
-                    help="run one extra measurement alone (profiling)")
+    ap.add_argument("--only", choices=("mfma", "stream", "save", "ttm", "c2"), default=None,
+                    help="run one extra measurement alone (profiling; save: SaveResults of 3D-size results; "
+                         "ttm: time to modes on --ttm-members members; c2: the FP32 form on the 2D operator)")
+    ap.add_argument("--save-dir", default="/dev/shm/rsvdt_save_bench",
+                    help="folder for --only save (removed afterwards)")
+    ap.add_argument("--save-nwe", type=int, default=32, help="frequency blocks of the --only save results")
+    ap.add_argument("--save-opt", type=int, default=2, choices=(1, 2))
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (profiler counter calibration)")
     args = ap.parse_args()
@@ -415,6 +513,15 @@ def main():
         if L.rsvdt_hip_init(0) != 0:
             raise RuntimeError("no HIP device")
         emit({"mfma_kernels": mfma_kernels(pkg, 1966080, 32, 32, 64)})
+        return
+    if args.only == "ttm":
+        emit({"time_to_modes": time_to_modes_gl(pkg, args.ttm_members)})
+        return
+    if args.only == "c2":
+        emit({"fp32_form_c2": fp32_form_c2(pkg)})
+        return
+    if args.only == "save":
+        emit({"save_results": save_results_c3(pkg, args.save_dir, nwe=args.save_nwe, opt=args.save_opt)})
         return
     if args.only == "stream":
         L = pkg.lib()
@@ -511,6 +618,9 @@ def main():
     if not args.no_ttm and not args.no_extras:
         out["time_to_modes"] = time_to_modes_gl(pkg, args.ttm_members)
         log(f"time to modes done ({time.time() - t_start:.0f} s)")
+    if not args.no_c2 and not args.no_extras:
+        out["fp32_form_c2"] = fp32_form_c2(pkg)
+        log(f"FP32 form on the 2D operator done ({time.time() - t_start:.0f} s)")
     out["bench_wall_s"] = round(time.time() - t_start, 1)
     emit(out)
 

@@ -59,7 +59,7 @@ __device__ __forceinline__ void tst(double2 *p, double2 v, bool nt) {
     else *p = v;
 }
 
-template <typename T, bool AREAL, int KC, int RPT, int UB, int MNZ>
+template <typename T, bool AREAL, int KC, int RPT, int UB>
 __global__ __launch_bounds__(256) void k_horner_tile(
     int nrows, int k, int hmax, int emax, int xcd_remap, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ tptr, const int32_t *__restrict__ tcol,
@@ -107,15 +107,6 @@ __global__ __launch_bounds__(256) void k_horner_tile(
     // staged through the caches: no nontemporal hint on it then
     const bool nty = (const void *)yv != (const void *)vin;
 
-    // each thread's rows' nonzeros (slot, value), the first MNZ of them
-    // copied to registers once per tile, so that the reads of the staged
-    // values of a pass have no dependent LDS load in front of them
-    constexpr int MZ = MNZ > 0 ? MNZ : 1;
-    int rb[RPT], re[RPT];
-    int rsl[RPT][MZ];
-    AV rval[RPT][MZ];
-    bool meta_regs = MNZ == 0;
-
     for (int j0 = 0; j0 < k; j0 += KC) {
         const int j = j0 + lane;
         const bool jok = j < k;
@@ -132,23 +123,6 @@ __global__ __launch_bounds__(256) void k_horner_tile(
             }
         }
         __syncthreads();        // metadata visible; the previous pass's reads of sx are done
-        if (!meta_regs) {
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int lr = rs + q * NS;
-                const bool ok = r0 + lr < r1;
-                rb[q] = ok ? srp[lr] : 0;
-                re[q] = ok ? srp[lr + 1] : 0;
-#pragma unroll
-                for (int u = 0; u < MZ; ++u) {
-                    const int e = rb[q] + u;
-                    rsl[q][u] = e < re[q] ? (int)sli[e] : 0;
-                    if (e < re[q]) rval[q][u] = sv[e];
-                    else rval[q][u] = AV{};
-                }
-            }
-            meta_regs = true;
-        }
         // stage the tile's distinct rows of vin, columns j0 .. j0+KC-1
         for (int i0 = rs; i0 < h; i0 += NS * UB) {
             V xb[UB];
@@ -171,30 +145,8 @@ __global__ __launch_bounds__(256) void k_horner_tile(
             const int row = r0 + lr;
             if (row >= r1) continue;
             double sr = 0, si = 0;
-            int eb, ee;
-            if constexpr (MNZ > 0) {
-                eb = rb[q];
-                ee = re[q];
-                V xs[MZ];
-#pragma unroll
-                for (int u = 0; u < MZ; ++u) xs[u] = sx[rsl[q][u] * KC + lane];   // slot 0 past the row's end
-#pragma unroll
-                for (int u = 0; u < MZ; ++u) {      // past the row's end the value is 0
-                    const double xr = (double)xs[u].x, xi = (double)xs[u].y;
-                    const AV a = rval[q][u];
-                    if constexpr (AREAL) {
-                        sr += a * xr;
-                        si += a * xi;
-                    } else {
-                        sr += a.x * xr - a.y * xi;
-                        si += a.x * xi + a.y * xr;
-                    }
-                }
-            } else {
-                eb = srp[lr];
-                ee = srp[lr + 1];
-            }
-            for (int e = eb + MNZ; e < ee; ++e) {     // the nonzeros not held in registers
+            const int eb = srp[lr], ee = srp[lr + 1];
+            for (int e = eb; e < ee; ++e) {
                 const V x = sx[(int)sli[e] * KC + lane];
                 const double xr = (double)x.x, xi = (double)x.y;
                 const AV a = sv[e];
@@ -244,7 +196,7 @@ static int launch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *y,
     const unsigned gx = ceil_div(d->nrows, TR);
     const unsigned gy = (unsigned)d->nb;
     const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
-    hipLaunchKernelGGL((k_horner_tile<T, AREAL, KC, RPT, 8, (RPT == 1 ? (AREAL ? 12 : 8) : 0)>), dim3(gx, gy), dim3(256), lay.total, s,
+    hipLaunchKernelGGL((k_horner_tile<T, AREAL, KC, RPT, 8>), dim3(gx, gy), dim3(256), lay.total, s,
                        (int)d->nrows, (int)d->k, d->tile_hmax, d->tile_emax, remap, d->nrows * d->k, d->rowptr,
                        d->tptr, d->tcol, d->lidx, d->val, (const V *)vin, (const V *)y, (const double2 *)ys,
                        (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
