@@ -346,6 +346,39 @@ def calib_copy(pkg, gib=1, reps=5):
                            "kernel": "k_zcopy3"}}
 
 
+def full_run(pkg, name="flow3d", precision=64):
+    """The whole main.c sequence (SourceCode/main.c:134-146: direct action,
+    one power iteration, SVD, StoreU, adjoint action, SVD) on a named workload
+    with transient removal at TL = T/2 (the stepping workload's setting), on
+    a seeded device sketch: wall time and, per action, the device time of the
+    stage kernels, the forcing GEMMs and the transient removal."""
+    A, k, Nw, w, dt, desc = workload(name)
+    T = 2 * np.pi / w
+    s = pkg.Session(A, k=k, q=1, Nw=Nw, w=w, TL=T / 2, dt=dt, transient_removal=True, precision=precision)
+    s.random_sketch(seed=1373, gaussian=True)
+    pkg.lib().rsvdt_hip_sync()
+    t0 = time.time()
+    actions = []
+    for nm, fn in (("direct", s.direct), ("power iteration", s.power_iteration), ("svd", s.svd_before_adjoint),
+                   ("store U", s.store_u), ("adjoint", s.adjoint), ("svd all", s.svd_allfreqs)):
+        fn()
+        pkg.lib().rsvdt_hip_sync()
+        if nm in ("direct", "adjoint"):
+            st = pkg.action_stats()
+            actions.append({"action": nm, "stage_kernels_s": round(st["rk4_ms"] / 1e3, 3),
+                            "forcing_s": round(st["forcing_ms"] / 1e3, 3),
+                            "transient_removal_s": round(st["tr_ms"] / 1e3, 3),
+                            "action_device_s": round(st["total_ms"] / 1e3, 3)})
+        log(f"{name} FP{precision} {nm} done at {time.time() - t0:.1f} s")
+    wall = time.time() - t0
+    S = pkg.mat_to_dense(pkg._vp(s.Res.S_hat)).real
+    s.close()
+    pkg.lib().RSVDT_Finalize()
+    return {"workload": f"{desc}, k={k}, Nw={Nw}, q=1, transient removal, TL=T/2, "
+                        + ("FP64" if precision == 64 else "FP32 state"),
+            "time_to_modes_s": round(wall, 2), "actions": actions, "sigma1_max": float(S[0].max())}
+
+
 def subspace_gap(U1, U2):
     """sin of the largest principal angle between span(U1) and span(U2)
     (orthonormal columns)."""
@@ -480,7 +513,7 @@ def main():
     ap.add_argument("--no-ttm", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the FP32-form measurement on the 2D operator")
     ap.add_argument("--ttm-members", type=int, default=50)
-    ap.add_argument("--only", choices=("mfma", "stream", "save", "ttm", "c2"), default=None,
+    ap.add_argument("--only", choices=("mfma", "stream", "save", "ttm", "c2", "full3d"), default=None,
                     help="run one extra measurement alone (profiling; save: SaveResults of 3D-size results; "
                          "ttm: time to modes on --ttm-members members; c2: the FP32 form on the 2D operator)")
     ap.add_argument("--save-dir", default="/dev/shm/rsvdt_save_bench",
@@ -516,6 +549,9 @@ def main():
         return
     if args.only == "ttm":
         emit({"time_to_modes": time_to_modes_gl(pkg, args.ttm_members)})
+        return
+    if args.only == "full3d":
+        emit({"full_run_flow3d": full_run(pkg, "flow3d")})
         return
     if args.only == "c2":
         emit({"fp32_form_c2": fp32_form_c2(pkg)})

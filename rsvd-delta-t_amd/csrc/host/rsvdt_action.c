@@ -157,6 +157,61 @@ static void multi_block_shape(rsvdt_spmm_desc *d, PetscInt k)
     d->lanes = l / 2 >= 4 ? l / 2 : 4;
 }
 
+/* Shape of the FP64 Horner stage on nb > 1 blocks of k columns (A or A^H):
+ * multi_block_shape's gather shape or the tiled kernel with 8 or 16 columns
+ * per pass, whichever three launches on the caller's buffers (x gathered,
+ * out written) time fastest; kept on the Mat for blocks of k columns.  An
+ * explicit SpMMVariant or SpMMLanes option keeps multi_block_shape. */
+static void multi_block_tuned(rsvdt_spmm_desc *d, Mat A, int dir, PetscInt k, const void *x, void *out)
+{
+    multi_block_shape(d, k);
+    if (g_rsvdt.opt.rk4_form || g_rsvdt.opt.spmm_variant >= 0 || g_rsvdt.opt.spmm_lanes || !g_rsvdt.opt.autotune)
+        return;
+    const int q = dir ? 0 : 1;
+    if (!(A->mt_ok[q] && A->mt_k[q] == k)) {
+        static const int cand[][2] = {{0, 0}, {8, 32}, {16, 16}, {16, 32}};
+        void *e0 = NULL, *e1 = NULL;
+        if (rsvdt_hip_event_create(&e0) || rsvdt_hip_event_create(&e1)) {
+            rsvdt_hip_event_destroy(e0);
+            return;
+        }
+        float best = 1e30f;
+        int bl = d->lanes, bv = d->variant, bt = 0;
+        for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
+            rsvdt_spmm_desc t = *d;
+            if (cand[c][0]) {
+                if (cand[c][0] > 8 && k <= cand[c][0] / 2) continue;
+                t.lanes = cand[c][0];
+                t.variant = 9;
+                if (!tile_attach(&t, A, dir, cand[c][1])) continue;
+            }
+            if (rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream)) continue;
+            rsvdt_hip_event_record(e0, g_rsvdt.stream);
+            for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream);
+            rsvdt_hip_event_record(e1, g_rsvdt.stream);
+            float ms = 0;
+            rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+            if (ms < best) { best = ms; bl = t.lanes; bv = t.variant; bt = cand[c][1]; }
+        }
+        rsvdt_hip_event_destroy(e0);
+        rsvdt_hip_event_destroy(e1);
+        A->mt_ok[q] = 1; A->mt_k[q] = k; A->mt_lanes[q] = bl; A->mt_variant[q] = bv; A->mt_tile[q] = bt;
+        if (g_rsvdt.opt.verbose)
+            rsvdt_printf("[rsvdt] autotune (%s, FP64, %lld blocks of %lld columns): lanes %d, variant %d, tile %d rows: "
+                         "%.1f us per stage\n", dir ? "A" : "A^H", (long long)d->nb, (long long)k, bl, bv, bt,
+                         1e3 * best / 3);
+    }
+    if (A->mt_variant[q] == 9) {
+        rsvdt_spmm_desc t = *d;
+        t.lanes = A->mt_lanes[q];
+        t.variant = 9;
+        if (tile_attach(&t, A, dir, A->mt_tile[q])) *d = t;
+    } else {
+        d->lanes = A->mt_lanes[q];
+        d->variant = A->mt_variant[q];
+    }
+}
+
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
  * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
  * while the two chunk buffers would take more than 60 % of free memory (an
@@ -341,9 +396,9 @@ static PetscErrorCode folded_forcing(Mat A, int dir, double h, PetscInt k, Petsc
         const PetscInt g = grp < Nwe - b0 ? grp : Nwe - b0;
         rsvdt_spmm_desc d;
         desc_for(&d, A, dir, 64, k, g);
-        multi_block_shape(&d, k);
         const double *Fb = F + 2 * (size_t)b0 * (size_t)N * (size_t)k;
         double *Gb = G + 2 * (size_t)b0 * (size_t)N * (size_t)k;
+        multi_block_tuned(&d, A, dir, k, Fb, w1);
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, Fb, Fb, dsc + 2 * b0, NULL, h / 4, w1, NULL, q));
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, w1, Fb, dsc + 2 * (Nwe + b0), NULL, h, w2, NULL, q));
         RSVDT_HIPCHK("TSActionRK4", rsvdt_hip_horner_stage(&d, w2, Fb, dsc + 2 * (2 * Nwe + b0), NULL, h * h / 6, Gb,
@@ -653,9 +708,10 @@ PetscErrorCode rsvdt_homogeneous_steps(Mat A, int dir, double h, PetscInt nsteps
         PetscInt g = G < nb - g0 ? G : nb - g0;
         rsvdt_spmm_desc d;
         desc_for(&d, A, dir, 64, k, g);
-        multi_block_shape(&d, k);
         double *y = X + 2 * (size_t)g0 * (size_t)N * (size_t)k;
         rsvdt_stream_t q = g_rsvdt.stream;
+        if (!fused && g > 1) multi_block_tuned(&d, A, dir, k, y, u1);
+        else multi_block_shape(&d, k);
         if (fused) {
             const double cf[4] = {h / 4, h / 3, h / 2, h};
             RSVDT_HIPCHK("TSTransRK4", rsvdt_hip_horner_fused(&d, A->nnz, dir ? A->maxrow : A->maxrowH, y, NULL, 0, cf,
