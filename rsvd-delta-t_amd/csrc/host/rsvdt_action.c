@@ -559,22 +559,19 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
  * rend = Nt + Ns (+RR) steps and stores Nstore snapshots, every RR steps
  * after Nt, into Y_all (block s = snapshot column s of every test vector;
  * the adjoint stores them reversed, SourceCode/SaveSnapshots.c:19-28). */
-static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat Fhat, int fhat_folded, Mat Yall,
-                                    PetscInt max_steps, phase_timer *t_rk4, phase_timer *t_frc)
+/* Steps i_first .. i_last (1-based, of 1 .. rend) of one stepping precision
+ * on the stepper's state, from forcing chunks that start at i_first. */
+static PetscErrorCode step_range(stepper *st, Mat A, RSVDt_vars *V, Mat Yall, PetscInt i_first, PetscInt i_last,
+                                 int fused, phase_timer *t_rk4, phase_timer *t_frc, int64_t *launches)
 {
-    RSVDT_CHK(rsvdt_aij_upload(A));
-    const int dir = V->TS.DirAdj ? 1 : 0;
-    const int prec = g_rsvdt.opt.precision;
-    const PetscInt k = V->RSVD.k, Nwe = V->RSVD.Nw_eff, N = A->rows;
+    const int dir = st->dir;
+    const PetscInt k = st->k, N = st->N;
     const PetscInt Nstore = V->TS.TransientRemoval ? V->RSVD.Nw + 1 : V->RSVD.Nw;
-    PetscInt rend = V->TS.Nt + V->TS.Ns + (V->TS.TransientRemoval ? V->TS.ResRatio : 0);
-    if (max_steps > 0 && max_steps < rend) rend = max_steps;
-    stepper st;
-    RSVDT_CHK(stepper_init(&st, A, dir, prec, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d));
-    /* From here on every failure is recorded in ierr and the loops stop, so
-     * that the tail (device sync, graph and event destruction) always runs:
-     * a forcing GEMM may still be writing a chunk buffer on the forcing
-     * stream when an error is met on the main stream. */
+    const PetscInt nstep = i_last - i_first + 1;
+    /* every failure is recorded in ierr and the loops stop, so that the tail
+     * (device sync, graph destruction) always runs: a forcing GEMM may still
+     * be writing a chunk buffer on the forcing stream when an error is met on
+     * the main stream */
     PetscErrorCode ierr = 0;
 #define TS_HIP(x)                                                                                      \
     do {                                                                                               \
@@ -583,59 +580,46 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
             if (_h) ierr = rsvdt_err(_h, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error()); \
         }                                                                                              \
     } while (0)
-    /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
-    const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, prec);
-    if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
-        ierr = autotune_lanes(A, dir, prec, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2);
-        desc_for(&st.desc, A, dir, prec, k, 1);
-        TS_HIP(rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream));
-    }
-    if (!ierr && st.form == 0 && !fhat_folded) {        /* G_m = Q(z_m) F_hat_m in place of F_hat */
-        if (t_frc) pt_begin(t_frc);
-        ierr = folded_forcing(A, dir, V->TS.dt, k, Nwe, V->TS.Ns, D->idft->d, Fhat->d, Fhat->d);
-        if (t_frc) pt_end(t_frc);
-    }
-    int64_t launches = 0;
-    const PetscInt nch = (rend + st.S - 1) / st.S;
+    const PetscInt nch = (nstep + st->S - 1) / st->S;
     /* forcing of the first two chunks; later chunks are refilled as soon as
      * the steps that read their buffer are done */
-    TS_HIP(rsvdt_hip_event_record(st.ev_rk[1], g_rsvdt.stream));   /* zero fill done */
+    TS_HIP(rsvdt_hip_event_record(st->ev_rk[1], g_rsvdt.stream));   /* state ready */
     for (PetscInt c = 0; c < nch && c < 2 && !ierr; c++) {
-        PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
-        TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[1]));
+        PetscInt i0 = i_first + c * st->S, steps = st->S < i_last - i0 + 1 ? st->S : i_last - i0 + 1;
+        TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st->ev_rk[1]));
         if (t_frc) pt_begin_on(t_frc, frc_stream());
-        if (!ierr) ierr = stepper_forcing(&st, (int)(c & 1), i0, steps);
+        if (!ierr) ierr = stepper_forcing(st, (int)(c & 1), i0, steps);
         if (t_frc) pt_end_on(t_frc, frc_stream());
-        TS_HIP(rsvdt_hip_event_record(st.ev_frc[c & 1], frc_stream()));
+        TS_HIP(rsvdt_hip_event_record(st->ev_frc[c & 1], frc_stream()));
     }
     /* graph replay of full chunks (form 0): one graph per buffer */
-    const int use_graph = g_rsvdt.opt.use_graph && st.form == 0 && !fused;
+    const int use_graph = g_rsvdt.opt.use_graph && st->form == 0 && !fused;
     void *gexec[2] = {NULL, NULL};
     int *slots = NULL;
     if (use_graph && !ierr) {
-        slots = rsvdt_scratch(SCR_SLOTS, sizeof(int) * 2 * (size_t)st.S);
+        slots = rsvdt_scratch(SCR_SLOTS, sizeof(int) * 2 * (size_t)st->S);
         if (!slots) ierr = rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate slot table");
     }
     for (PetscInt c = 0; c < nch && !ierr; c++) {
         const int buf = (int)(c & 1);
-        const PetscInt i0 = 1 + c * st.S, steps = st.S < rend - i0 + 1 ? st.S : rend - i0 + 1;
-        TS_HIP(rsvdt_hip_stream_wait_event(g_rsvdt.stream, st.ev_frc[buf]));
+        const PetscInt i0 = i_first + c * st->S, steps = st->S < i_last - i0 + 1 ? st->S : i_last - i0 + 1;
+        TS_HIP(rsvdt_hip_stream_wait_event(g_rsvdt.stream, st->ev_frc[buf]));
         if (ierr) break;
         if (t_rk4) pt_begin(t_rk4);
         if (fused) {
-            const double cf[4] = {st.h / 4, st.h / 3, st.h / 2, st.h};
-            TS_HIP(rsvdt_hip_horner_fused(&st.desc, A->nnz, dir ? A->maxrow : A->maxrowH, st.y, fslot(&st, buf, 0),
+            const double cf[4] = {st->h / 4, st->h / 3, st->h / 2, st->h};
+            TS_HIP(rsvdt_hip_horner_fused(&st->desc, A->nnz, dir ? A->maxrow : A->maxrowH, st->y, fslot(st, buf, 0),
                                           (int64_t)N * k, cf, steps, Yall ? Yall->d : NULL, (int64_t)N * k, i0,
                                           V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream));
-            launches += 1;
-        } else if (use_graph && steps == st.S) {
-            int *sl = slots + buf * st.S;
-            if (Yall) TS_HIP(rsvdt_hip_snap_slots(sl, st.S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream));
-            if (!ierr && !gexec[buf]) ierr = stepper_capture(&st, buf, sl, Yall ? Yall->d : NULL, &gexec[buf]);
+            *launches += 1;
+        } else if (use_graph && steps == st->S) {
+            int *sl = slots + buf * st->S;
+            if (Yall) TS_HIP(rsvdt_hip_snap_slots(sl, st->S, i0, V->TS.Nt, V->TS.ResRatio, Nstore, dir, g_rsvdt.stream));
+            if (!ierr && !gexec[buf]) ierr = stepper_capture(st, buf, sl, Yall ? Yall->d : NULL, &gexec[buf]);
             TS_HIP(rsvdt_hip_graph_launch(gexec[buf], g_rsvdt.stream));
-            launches += 4 * steps;
+            *launches += 4 * steps;
         }
-        for (PetscInt s = 0; s < steps && !ierr && !fused && !(use_graph && steps == st.S); s++) {
+        for (PetscInt s = 0; s < steps && !ierr && !fused && !(use_graph && steps == st->S); s++) {
             PetscInt i = i0 + s;
             double *snap = NULL;
             if (Yall && i > V->TS.Nt && (i - V->TS.Nt) % V->TS.ResRatio == 0) {
@@ -644,28 +628,90 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
                 if (!dir) pos = Nstore - 1 - pos;
                 snap = Yall->d + 2 * (size_t)pos * (size_t)N * (size_t)k;
             }
-            ierr = stepper_step(&st, buf, s, snap);
-            launches += 4;
+            ierr = stepper_step(st, buf, s, snap);
+            *launches += 4;
         }
         if (t_rk4) pt_end(t_rk4);
-        TS_HIP(rsvdt_hip_event_record(st.ev_rk[buf], g_rsvdt.stream));
+        TS_HIP(rsvdt_hip_event_record(st->ev_rk[buf], g_rsvdt.stream));
         if (c + 2 < nch && !ierr) {
-            const PetscInt j0 = 1 + (c + 2) * st.S, st2 = st.S < rend - j0 + 1 ? st.S : rend - j0 + 1;
-            TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st.ev_rk[buf]));
+            const PetscInt j0 = i_first + (c + 2) * st->S, st2 = st->S < i_last - j0 + 1 ? st->S : i_last - j0 + 1;
+            TS_HIP(rsvdt_hip_stream_wait_event(frc_stream(), st->ev_rk[buf]));
             if (t_frc) pt_begin_on(t_frc, frc_stream());
-            if (!ierr) ierr = stepper_forcing(&st, buf, j0, st2);
+            if (!ierr) ierr = stepper_forcing(st, buf, j0, st2);
             if (t_frc) pt_end_on(t_frc, frc_stream());
-            TS_HIP(rsvdt_hip_event_record(st.ev_frc[buf], frc_stream()));
+            TS_HIP(rsvdt_hip_event_record(st->ev_frc[buf], frc_stream()));
         }
     }
 #undef TS_HIP
     /* the main stream must not run ahead of a forcing GEMM still reading
      * F_hat, and no buffer may be reused while one is writing: always wait */
-    {
-        int hs = rsvdt_hip_sync();
-        if (hs && !ierr) ierr = rsvdt_err(hs, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
-    }
+    int hs = rsvdt_hip_sync();
+    if (hs && !ierr) ierr = rsvdt_err(hs, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
     for (int b = 0; b < 2; b++) rsvdt_hip_graph_destroy(gexec[b]);
+    return ierr;
+}
+
+/* The time stepping of one action (TSActionRK4.c:38-130) over rend steps.
+ * With the FP32 state and transient removal, the Nt steps before the first
+ * snapshot run in FP32 and the snapshot period in FP64: the FP32 rounding
+ * accumulated by then is a perturbation of the state at step Nt, i.e. part
+ * of the transient, which the FP64 snapshots carry like the rest of it and
+ * the removal takes out (kept in FP32, the removal's correction of the
+ * transient would carry the rounding into the results). */
+static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat Fhat, int fhat_folded, Mat Yall,
+                                    PetscInt max_steps, phase_timer *t_rk4, phase_timer *t_frc)
+{
+    RSVDT_CHK(rsvdt_aij_upload(A));
+    const int dir = V->TS.DirAdj ? 1 : 0;
+    const int prec = g_rsvdt.opt.precision;
+    const PetscInt k = V->RSVD.k, Nwe = V->RSVD.Nw_eff, N = A->rows;
+    PetscInt rend = V->TS.Nt + V->TS.Ns + (V->TS.TransientRemoval ? V->TS.ResRatio : 0);
+    if (max_steps > 0 && max_steps < rend) rend = max_steps;
+    const int split = prec == 32 && V->TS.TransientRemoval && Yall && V->TS.Nt > 0 && rend > V->TS.Nt;
+    const PetscInt seg_last[2] = {split ? V->TS.Nt : rend, rend};
+    const int seg_prec[2] = {prec, 64};
+    int64_t launches = 0;
+    PetscErrorCode ierr = 0;
+    double *y64 = NULL;          /* the FP32 state at step Nt, widened */
+    stepper st;
+    memset(&st, 0, sizeof(st));
+    for (int seg = 0; seg < (split ? 2 : 1) && !ierr; seg++) {
+        const int p = seg_prec[seg];
+        ierr = stepper_init(&st, A, dir, p, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d);
+        if (ierr) break;
+        /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
+        const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, p);
+        if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
+            ierr = autotune_lanes(A, dir, p, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2);
+            desc_for(&st.desc, A, dir, p, k, 1);
+            int hz = ierr ? 0 : rsvdt_hip_zero(st.y, (size_t)N * k * st.esz, g_rsvdt.stream);
+            if (hz) ierr = rsvdt_err(hz, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+        }
+        if (!ierr && seg == 1) {
+            int hc = rsvdt_hip_memcpy_d2d(st.y, y64, (size_t)N * (size_t)k * 16, g_rsvdt.stream);
+            if (hc) ierr = rsvdt_err(hc, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+        }
+        if (!ierr && seg == 0 && st.form == 0 && !fhat_folded) {   /* G_m = Q(z_m) F_hat_m in place of F_hat */
+            if (t_frc) pt_begin(t_frc);
+            ierr = folded_forcing(A, dir, V->TS.dt, k, Nwe, V->TS.Ns, D->idft->d, Fhat->d, Fhat->d);
+            if (t_frc) pt_end(t_frc);
+        }
+        if (!ierr)
+            ierr = step_range(&st, A, V, Yall, seg ? seg_last[0] + 1 : 1, seg_last[seg], fused, t_rk4, t_frc,
+                              &launches);
+        if (!ierr && split && seg == 0) {
+            if (rsvdt_hip_malloc((void **)&y64, (size_t)N * (size_t)k * 16))
+                ierr = rsvdt_err(RSVDT_ERR_MEM, "TSActionRK4", "cannot allocate the FP64 state: %s",
+                                 rsvdt_hip_last_error());
+            int hc = ierr ? 0 : rsvdt_hip_c2z((int64_t)N * k, (const float *)st.y, y64, g_rsvdt.stream);
+            if (hc) ierr = rsvdt_err(hc, "TSActionRK4", "device call failed: %s", rsvdt_hip_last_error());
+        }
+        if (seg == 0 && split) stepper_free(&st);
+    }
+    if (y64) {
+        rsvdt_hip_sync();
+        rsvdt_hip_free(y64);
+    }
     stepper_free(&st);
     /* in an action, chunk buffers of more than 1 GiB go back to the device:
      * the transient removal that follows sizes its groups to the free memory */

@@ -750,6 +750,7 @@ PetscErrorCode rsvdt_aij_upload(Mat m)
     m->h_colH = ci;
     free(v); free(fill);
     memset(m->tuned_ok, 0, sizeof(m->tuned_ok));
+    memset(m->mt_ok, 0, sizeof(m->mt_ok));
     return ierr;
 }
 

@@ -153,14 +153,11 @@ PetscErrorCode ReadUserInput(RSVDt_vars *RSVDt, Weight_matrices *W, LNS_vars *LN
         else return rsvdt_err(RSVDT_ERR_USER, "ReadUserInput", "'RK4Form' must be horner or stages");
     }
     if (o.precision == 32 && RSVDt->TS.TransientRemoval) {
-        /* the removal's Galerkin correction of the transient left at
-         * TransientLength carries the FP32 state's rounding into Y_hat: on the
-         * 2D flow operator (abscissa -0.05, T = 20) the gains differ from the
-         * FP64 form by up to 4e-5 at TL = T/2 and 2e-8 at TL = 6T */
-        rsvdt_printf("Warning: 'Precision: fp32' with transient removal: the removal corrects the transient left at "
-                     "TransientLength and carries the FP32 rounding of the state into the results in proportion to "
-                     "it; with a TransientLength of several periods the gains match the FP64 form to about 1e-8, with "
-                     "a fraction of a period they can differ by 1e-5 or more on slowly decaying operators\n");
+        /* kept in FP32, the snapshots would carry the state's rounding into
+         * the removal's correction of the transient (on the 2D flow operator
+         * the gains then differ from the FP64 form by up to 4e-5) */
+        rsvdt_printf("Note: 'Precision: fp32' with transient removal: the steps before the first snapshot run in FP32, "
+                     "the snapshot period in FP64\n");
     }
     RSVDT_CHK(RSVDT_SetOptions(&o));
     rsvdt_printf("\n");

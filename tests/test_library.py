@@ -175,7 +175,8 @@ def test_read_user_input_flag_dirs(pkg, tmp_path, capfd):
     rc, V, W, LNS, TR, D = read_user_input(pkg, p)
     assert rc == 0 and LNS.RSVDt.Disc.DiscFlg == 1 and LNS.RSVDt.Disc.beta == 0.1
     assert pkg.get_options().precision == 32
-    assert "'Precision: fp32' with transient removal" in capfd.readouterr().out   # TransientRemoval is on
+    assert "'Precision: fp32' with transient removal: the steps before the first snapshot run in FP32" in \
+        capfd.readouterr().out                                                      # TransientRemoval is on
     pkg.set_options(precision=64)
 
 
