@@ -95,96 +95,184 @@ __global__ __launch_bounds__(256) void k_bchol_inv(int n, const double2 *__restr
     if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
 }
 
-// The same with the factor and its inverse held in LDS for n <= 70 (2*n*n*16
-// bytes of dynamic LDS): every read of the elimination and of the triangular inverse stays on
-// the CU; the global arrays are read once and written once.  The inverse is
-// formed row by row from the bottom, all columns of a row in parallel:
-// X(i, c) = -(sum_{l=i+1..c} R(i,l) X(l,c)) / R(i,i), X(c, c) = 1 / R(c,c).
-__global__ __launch_bounds__(256) void k_bchol_inv_lds(int n, const double2 *__restrict__ G, double shift_rel,
+// The same for n <= 16 NB (NB <= 5) with the matrix in registers: thread
+// (tr, tc) = (tid / 16, tid % 16) of the 16 x 16 grid owns the entries
+// (tr + 16a, tc + 16b), so that every step's work stays spread over the
+// whole workgroup as the trailing block shrinks; the blocks a > b lie below
+// the diagonal for every thread and are not held.  Cholesky, right-looking:
+// at step j the owners of row j put it in LDS (double-buffered, so one
+// barrier per step), every thread takes the pivot from there and updates its
+// entries with R(i, c) -= conj(R(j, i)) R(j, c) / d unconditionally, the
+// multipliers being zero outside the trailing block, so the step has no
+// branches; row j keeps its unscaled values and is scaled once at the end
+// from the pivots kept in LDS.  Inverse, solving R X = I from the bottom row
+// up the same way: at step i the owners of column i of R and of row i of the
+// right-hand side put them in LDS, one barrier, B(r, :) -= R(r, i) X(i, :)
+// for r < i with X(i, :) = B(i, :) / R(i, i) from the kept reciprocals.
+// 2n barriers, no division inside the steps but the pivot's own.  The steps
+// run in blocks of 16 with the block index known at compile time, so the
+// finished blocks drop out of the update and the owners of the broadcast
+// row or column store it without per-element guards.
+__device__ __forceinline__ void zfms(double2 &w, double2 l, double2 u)
+{
+    // w -= l * u
+    w.x = fma(-l.x, u.x, fma(l.y, u.y, w.x));
+    w.y = fma(-l.x, u.y, fma(-l.y, u.x, w.y));
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_bchol_inv_reg(int n, const double2 *__restrict__ G, double shift_rel,
                                                        double drop_tol, double2 *__restrict__ R,
                                                        double2 *__restrict__ Rinv, int *info)
 {
-    extern __shared__ double2 cl[];          // R then X, n*n each
-    double2 *sR = cl, *sX = cl + n * n;
-    __shared__ double s_piv, s_shift, s_dmax;
-    __shared__ int s_bad;
+    constexpr int NP = 16 * NB;
+    __shared__ double2 rowb[2][NP], colb[2][NP];
+    __shared__ double s_diag[NP], s_inv[NP], s_piv[NP];
+    __shared__ double s_shift, s_dmax;
+    const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
     const int64_t b = blockIdx.x;
     const double2 *Gb = G + b * n * n;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e / n, c = e % n;
-        sR[e] = (i <= c) ? Gb[e] : make_double2(0, 0);
-    }
+    double2 Rr[NB][NB];                      // [a][q], a <= q used
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int q = a; q < NB; ++q) {
+            const int i = tr + 16 * a, c = tc + 16 * q;
+            Rr[a][q] = (i < n && c < n && i <= c) ? Gb[i * n + c] : make_double2(0, 0);
+            if (i == c && i < n) s_diag[i] = Rr[a][q].x;
+        }
     __syncthreads();
-    if (threadIdx.x < 64) {                  // trace and largest diagonal entry: one wave
-        double tr = 0, dm = 0;
-        for (int i = threadIdx.x; i < n; i += 64) { tr += sR[i * n + i].x; dm = fmax(dm, sR[i * n + i].x); }
+    if (tid < 64) {                          // trace and largest diagonal entry: one wave
+        double tr2 = 0, dm = 0;
+        for (int i = tid; i < n; i += 64) { tr2 += s_diag[i]; dm = fmax(dm, s_diag[i]); }
         for (int o = 32; o > 0; o >>= 1) {
-            tr += __shfl_xor(tr, o, 64);
+            tr2 += __shfl_xor(tr2, o, 64);
             dm = fmax(dm, __shfl_xor(dm, o, 64));
         }
-        if (threadIdx.x == 0) { s_shift = shift_rel * tr; s_dmax = dm; s_bad = 0; }
+        if (tid == 0) { s_shift = shift_rel * tr2; s_dmax = dm; }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double2 v = sR[i * n + i];
-        v.x += s_shift;
-        v.y = 0;
-        sR[i * n + i] = v;
+    const double shift = s_shift, dmax = s_dmax;
+    if (tr == tc) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) { Rr[a][a].x += shift; Rr[a][a].y = 0; }
     }
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        if (threadIdx.x == 0) {
-            double d = sR[j * n + j].x;
-            if (drop_tol > 0 && !(d > drop_tol * s_dmax)) {
-                s_piv = 0;                      // column j lies in the span of columns < j: drop it
+    int bad = 0;
+    // steps in blocks of 16 (a0 at compile time): row j = 16 a0 + t lives in
+    // register block a0, and the blocks above a0 are finished
+#pragma unroll
+    for (int a0 = 0; a0 < NB; ++a0) {
+        const int jend = n < 16 * (a0 + 1) ? n : 16 * (a0 + 1);
+        for (int j = 16 * a0; j < jend; ++j) {
+            const int buf = j & 1;
+            if (tr == j - 16 * a0) {         // row j (its columns below j are not read)
+#pragma unroll
+                for (int q = a0; q < NB; ++q) rowb[buf][tc + 16 * q] = Rr[a0][q];
+            }
+            __syncthreads();
+            double d = rowb[buf][j].x, piv;
+            if (drop_tol > 0 && !(d > drop_tol * dmax)) {
+                piv = 0;                        // column j lies in the span of columns < j: drop it
             } else {
-                if (!(d > 1e-300)) { d = 1e-300; s_bad++; }
-                s_piv = sqrt(d);
+                if (!(d > 1e-300)) { d = 1e-300; bad++; }
+                piv = sqrt(d);
             }
-            sR[j * n + j] = make_double2(s_piv, 0);
-        }
-        __syncthreads();
-        const double inv = s_piv > 0 ? 1.0 / s_piv : 0.0;
-        for (int c = j + 1 + threadIdx.x; c < n; c += blockDim.x) {
-            const double2 v = sR[j * n + c];
-            sR[j * n + c] = make_double2(v.x * inv, v.y * inv);
-        }
-        __syncthreads();
-        const int m = n - j - 1;
-        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-            const int i = j + 1 + e / m, c = j + 1 + e % m;
-            if (i <= c) {
-                const double2 ri = sR[j * n + i], rc = sR[j * n + c];
-                double2 v = sR[i * n + c];
-                v.x -= ri.x * rc.x + ri.y * rc.y;
-                v.y -= ri.x * rc.y - ri.y * rc.x;
-                sR[i * n + c] = v;
+            const double inv = piv > 0 ? 1.0 / piv : 0.0;
+            if (tid == 0) { s_piv[j] = piv; s_inv[j] = inv; }
+            const double inv2 = inv * inv;
+            double2 ri[NB], rc[NB];
+#pragma unroll
+            for (int a = a0; a < NB; ++a) {
+                const int i = tr + 16 * a;
+                const double2 v = rowb[buf][i];
+                const bool on = i > j && i < n;
+                ri[a] = make_double2(on ? v.x * inv2 : 0.0, on ? -v.y * inv2 : 0.0);   // conj(R(j, i)) / d
             }
+#pragma unroll
+            for (int q = a0; q < NB; ++q) {
+                const int c = tc + 16 * q;
+                const double2 v = rowb[buf][c];
+                const bool on = c > j && c < n;
+                rc[q] = make_double2(on ? v.x : 0.0, on ? v.y : 0.0);
+            }
+#pragma unroll
+            for (int a = a0; a < NB; ++a)
+#pragma unroll
+                for (int q = a; q < NB; ++q) zfms(Rr[a][q], ri[a], rc[q]);
         }
-        __syncthreads();
     }
-    // inverse, bottom row first; dropped columns (zero pivot) give zero rows and columns
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) sX[e] = make_double2(0, 0);
-    __syncthreads();
-    for (int i = n - 1; i >= 0; --i) {
-        const double rii = sR[i * n + i].x;
-        for (int c = i + threadIdx.x; c < n; c += blockDim.x) {
-            double2 v = make_double2(0, 0);
-            if (rii > 0) {
-                if (c == i) v = make_double2(1.0 / rii, 0);
-                else if (sR[c * n + c].x > 0) {
-                    double2 acc = make_double2(0, 0);
-                    for (int l = i + 1; l <= c; ++l) acc = zadd(acc, zmul(sR[i * n + l], sX[l * n + c]));
-                    v = make_double2(-acc.x / rii, -acc.y / rii);
-                }
-            }
-            sX[i * n + c] = v;
+    __syncthreads();                         // pivots and reciprocals visible; row buffers free
+    // R(i, i) = piv_i, R(i, c) *= inv_i (c > i)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+        const int i = tr + 16 * a;
+        const double inv = i < n ? s_inv[i] : 0.0, piv = i < n ? s_piv[i] : 0.0;
+#pragma unroll
+        for (int q = a; q < NB; ++q) {
+            const int c = tc + 16 * q;
+            Rr[a][q] = c == i ? make_double2(piv, 0) : make_double2(Rr[a][q].x * inv, Rr[a][q].y * inv);
         }
-        __syncthreads();
+    }
+    // X = R^-1 from R X = I, bottom row first; B holds X's rows unscaled
+    double2 Br[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int q = a; q < NB; ++q)
+            Br[a][q] = make_double2((tr + 16 * a == tc + 16 * q && tr + 16 * a < n) ? 1.0 : 0.0, 0);
+#pragma unroll
+    for (int a0 = NB - 1; a0 >= 0; --a0) {
+        const int iend = (n < 16 * (a0 + 1) ? n : 16 * (a0 + 1)) - 1;
+        for (int i = iend; i >= 16 * a0; --i) {
+            const int buf = i & 1;
+            if (tc == i - 16 * a0) {           // column i of R (its rows above i are read)
+#pragma unroll
+                for (int a = 0; a <= a0; ++a) colb[buf][tr + 16 * a] = Rr[a][a0];
+            }
+            if (tr == i - 16 * a0) {           // row i of B (its columns from i on are read)
+#pragma unroll
+                for (int q = a0; q < NB; ++q) rowb[buf][tc + 16 * q] = Br[a0][q];
+            }
+            __syncthreads();
+            const double inv = s_inv[i];       // 0 for a dropped column: zero row (and column) of X
+            double2 xi[NB], rr[NB];
+#pragma unroll
+            for (int q = a0; q < NB; ++q) {
+                const int c = tc + 16 * q;
+                const double2 v = rowb[buf][c];
+                const bool on = c >= i && c < n;
+                xi[q] = make_double2(on ? v.x * inv : 0.0, on ? v.y * inv : 0.0);
+            }
+#pragma unroll
+            for (int a = 0; a <= a0; ++a) {
+                const double2 v = colb[buf][tr + 16 * a];
+                const bool on = tr + 16 * a < i;
+                rr[a] = make_double2(on ? v.x : 0.0, on ? v.y : 0.0);
+            }
+#pragma unroll
+            for (int a = 0; a <= a0; ++a)
+#pragma unroll
+                for (int q = a0; q < NB; ++q) zfms(Br[a][q], rr[a], xi[q]);
+        }
     }
     double2 *Rb = R + b * n * n, *Xb = Rinv + b * n * n;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) { Rb[e] = sR[e]; Xb[e] = sX[e]; }
-    if (threadIdx.x == 0 && info != nullptr && s_bad) atomicAdd(info, s_bad);
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+        const int i = tr + 16 * a;
+        const double inv = i < n ? s_inv[i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int c = tc + 16 * q;
+            if (i < n && c < n) {
+                const bool up = q >= a && i <= c;
+                const double2 rv = q >= a ? Rr[a][q] : make_double2(0, 0);
+                const double2 bv = q >= a ? Br[a][q] : make_double2(0, 0);
+                Rb[i * n + c] = up ? rv : make_double2(0, 0);
+                Xb[i * n + c] = up ? make_double2(bv.x * inv, bv.y * inv) : make_double2(0, 0);
+            }
+        }
+    }
+    if (tid == 0 && info != nullptr && bad) atomicAdd(info, bad);
 }
 
 // D_b = diag(1 where R_b(j,j) is not positive, else 0): the columns that a
@@ -416,6 +504,132 @@ __global__ __launch_bounds__(256) void k_bshift_solve(int n, int nrhs, const dou
     if (threadIdx.x == 0 && bad && info != nullptr) atomicAdd(info, 1);
 }
 
+// The same for n + 1 <= 16 NB (NB <= 5) with W = shift I - M_b and the
+// right-hand side (column n) in registers: thread (tr, tc) = (tid % 16,
+// tid / 16) owns the entries (tr + 16a, tc + 16b), so that the 16 owners of
+// a column are 16 adjacent lanes of one wave.  Gauss-Jordan elimination with
+// partial pivoting and no row exchanges: at step i the owners of column i
+// find the pivot row p among the rows not yet used (a 16-lane shuffle
+// reduction, lowest index on ties) and put column i in LDS; one barrier; the
+// owners of row p put it in LDS; one barrier; every row r != p takes
+// W(r, c) -= (W(r, i) / W(p, i)) W(p, c) for c > i.  At the end row p_i holds
+// only W(p_i, i) and its right-hand side, so x_i = W(p_i, n) / W(p_i, i).
+// 2n barriers, no back substitution; a zero pivot gives a zero component
+// and counts in info.  The steps run in blocks of 16 with the block index
+// known at compile time, so the finished column blocks drop out of the
+// update, and the update is branch-free (multipliers zeroed where unused).
+template <int NB>
+__global__ __launch_bounds__(256) void k_bshift_solve_reg(int n, int nrhs, const double2 *__restrict__ M,
+                                                          const double2 *__restrict__ shift, const double2 *B,
+                                                          double2 *X, int *info)
+{
+    constexpr int NP = 16 * NB;
+    __shared__ double2 colb[2][NP], rowb[2][NP], rhs[NP], pivv[NP];
+    __shared__ int s_p[2], perm[NP];
+    const int tid = threadIdx.x, tr = tid % 16, tc = tid / 16;
+    const int64_t sys = blockIdx.x;
+    const int64_t b = sys / nrhs;
+    const int m = (int)(sys % nrhs);
+    const double2 *Mb = M + b * n * n;
+    const double2 sh = shift[b * nrhs + m];
+    double2 W[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int r = tr + 16 * a, c = tc + 16 * q;
+            double2 w = make_double2(0, 0);
+            if (r < n && c < n) {
+                const double2 v = Mb[r * n + c];
+                w = make_double2(-v.x, -v.y);
+                if (r == c) { w.x += sh.x; w.y += sh.y; }
+            } else if (r < n && c == n) {
+                w = B[(b * n + r) * nrhs + m];
+            }
+            W[a][q] = w;
+        }
+    unsigned used = 0;                         // bit a: row tr + 16a is a pivot row
+    int bad = 0;
+    // steps in blocks of 16 (q0 at compile time): column i = 16 q0 + t lives
+    // in register block q0, and the blocks left of q0 are finished
+#pragma unroll
+    for (int q0 = 0; q0 < NB; ++q0) {
+        const int iend = n < 16 * (q0 + 1) ? n : 16 * (q0 + 1);
+        for (int i = 16 * q0; i < iend; ++i) {
+            const int buf = i & 1;
+            if (tc == i - 16 * q0) {           // column i: pivot among the rows not yet used
+                double best = -1;
+                int bi = n;
+#pragma unroll
+                for (int a = 0; a < NB; ++a) {
+                    const int r = tr + 16 * a;
+                    colb[buf][r] = W[a][q0];
+                    const double x = zabs2(W[a][q0]);
+                    const bool cand = r < n && !((used >> a) & 1) && x > best;
+                    best = cand ? x : best;
+                    bi = cand ? r : bi;
+                }
+                for (int o = 8; o > 0; o >>= 1) {
+                    const double v2 = __shfl_xor(best, o, 64);
+                    const int i2 = __shfl_xor(bi, o, 64);
+                    if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+                }
+                if (tr == 0) s_p[buf] = bi;
+            }
+            __syncthreads();
+            const int p = s_p[buf], ap = p / 16;
+            if (tr == p % 16) {                // row p (its columns up to i are not read)
+#pragma unroll
+                for (int a = 0; a < NB; ++a)
+                    if (a == ap) {
+#pragma unroll
+                        for (int q = q0; q < NB; ++q) rowb[buf][tc + 16 * q] = W[a][q];
+                    }
+                used |= 1u << ap;
+            }
+            __syncthreads();
+            const double2 d = colb[buf][p];
+            const double dd = zabs2(d);
+            if (!(dd > 0)) bad = 1;
+            const double2 dinv = dd > 0 ? make_double2(d.x / dd, -d.y / dd) : make_double2(0, 0);
+            if (tid == 0) { perm[i] = p; pivv[i] = dinv; }
+            double2 l[NB];
+#pragma unroll
+            for (int a = 0; a < NB; ++a) {
+                const int r = tr + 16 * a;
+                const double2 lv = zmul(colb[buf][r], dinv);
+                const bool on = r < n && r != p;
+                l[a] = make_double2(on ? lv.x : 0.0, on ? lv.y : 0.0);
+            }
+#pragma unroll
+            for (int q = q0; q < NB; ++q) {
+                const int c = tc + 16 * q;
+                const double2 v = rowb[buf][c];
+                const bool on = c > i && c <= n;
+                const double2 u = make_double2(on ? v.x : 0.0, on ? v.y : 0.0);
+#pragma unroll
+                for (int a = 0; a < NB; ++a) zfms(W[a][q], l[a], u);
+            }
+        }
+    }
+    // x_i = W(p_i, n) / W(p_i, i)
+    if (tc == n % 16) {
+        const int qn = n / 16;
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+            if (q == qn) {
+#pragma unroll
+                for (int a = 0; a < NB; ++a) {
+                    const int r = tr + 16 * a;
+                    if (r < n) rhs[r] = W[a][q];
+                }
+            }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) X[(b * n + i) * nrhs + m] = zmul(rhs[perm[i]], pivv[i]);
+    if (tid == 0 && bad && info != nullptr) atomicAdd(info, 1);
+}
+
 // The same with W and the right-hand side in LDS for n <= 96: the pivot
 // search is one wave's shuffle reduction, and each elimination step is two
 // barriers.  Row-major W(i, c) = shift*I - M_b.
@@ -510,10 +724,14 @@ extern "C" int rsvdt_hip_bchol_inv(int64_t nbatch, int64_t n, const double *G, d
 {
     if (nbatch <= 0 || n <= 0) return RSVDT_HIP_OK;
     if (n > 1024 || nbatch > 0x7fffffff) { set_error_msg("rsvdt_hip_bchol_inv: n <= 1024"); return RSVDT_HIP_ERR_ARG; }
-    if (n <= 70)      // R and R^-1 in LDS (2 n^2 x 16 B <= 154 KB)
-        hipLaunchKernelGGL(k_bchol_inv_lds, dim3((unsigned)nbatch), dim3(256), (size_t)2 * n * n * 16, as_stream(s),
-                           (int)n, (const double2 *)G, shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
-    else
+    const int nb = (int)((n + 15) / 16);
+#define RSVDT_CHOL_REG(NB)                                                                                   \
+    if (nb == NB) {                                                                                          \
+        hipLaunchKernelGGL(k_bchol_inv_reg<NB>, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n,  \
+                           (const double2 *)G, shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);   \
+    } else
+    RSVDT_CHOL_REG(1) RSVDT_CHOL_REG(2) RSVDT_CHOL_REG(3) RSVDT_CHOL_REG(4) RSVDT_CHOL_REG(5)
+#undef RSVDT_CHOL_REG
         hipLaunchKernelGGL(k_bchol_inv, dim3((unsigned)nbatch), dim3(256), 0, as_stream(s), (int)n,
                            (const double2 *)G, shift_rel, drop_tol, (double2 *)R, (double2 *)Rinv, info);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_bchol_inv");
@@ -562,6 +780,15 @@ extern "C" int rsvdt_hip_bshift_solve(int64_t nbatch, int64_t n, int64_t nrhs, c
 {
     if (nbatch <= 0 || n <= 0 || nrhs <= 0) return RSVDT_HIP_OK;
     if (n > 4096 || nbatch * nrhs > 0x7fffffffLL) { set_error_msg("rsvdt_hip_bshift_solve: bad size"); return RSVDT_HIP_ERR_ARG; }
+    const int nb = (int)((n + 16) / 16);      // n + 1 columns with the right-hand side
+#define RSVDT_SOLVE_REG(NB)                                                                                  \
+    if (nb == NB) {                                                                                          \
+        hipLaunchKernelGGL(k_bshift_solve_reg<NB>, dim3((unsigned)(nbatch * nrhs)), dim3(256), 0, as_stream(s), \
+                           (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift, (const double2 *)B,   \
+                           (double2 *)X, info);                                                              \
+    } else
+    RSVDT_SOLVE_REG(1) RSVDT_SOLVE_REG(2) RSVDT_SOLVE_REG(3) RSVDT_SOLVE_REG(4) RSVDT_SOLVE_REG(5)
+#undef RSVDT_SOLVE_REG
     if (n <= 96)      // W and the right-hand side in LDS ((n^2 + n) x 16 B <= 149 KB)
         hipLaunchKernelGGL(k_bshift_solve_lds, dim3((unsigned)(nbatch * nrhs)), dim3(256), (size_t)(n * n + n) * 16,
                            as_stream(s), (int)n, (int)nrhs, (const double2 *)M, (const double2 *)shift,
