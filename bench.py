@@ -69,7 +69,7 @@ def workload(name):
             "flow2d N=393216 (3 fields, 512x256 periodic, 11 nnz/row)")
 
 
-STAGE_KERNELS = {6: "k_horner_group", 7: "k_horner_pair", 8: "k_horner_pair", 9: "k_horner_tile"}
+STAGE_KERNELS = {6: "k_horner_group", 7: "k_horner_pair", 8: "k_horner_pair", 9: "k_horner_tile", 10: "k_horner_tile"}
 
 
 def stepping(pkg, name, steps, warmup, precision=64, chunk=64, rk4_form=0, graph=False):

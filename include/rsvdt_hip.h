@@ -98,7 +98,9 @@ typedef struct {
                              columns per lane (16-byte gathers; k/2 in 4, 8, 16, 32 lanes);
                              8: the same with two such pairs per lane (k/4 in 4, 8, 16 lanes);
                              9: the Horner stage on the tiled form below (lanes = columns per
-                             pass: 8, 16 or 32, 0 = automatic) */
+                             pass: 8, 16 or 32, 0 = automatic); 10: the same with the next
+                             pass's first distinct rows loaded into registers while the
+                             current pass computes */
     /* row-grouped form of the same operator (variant 6): groups of `group`
      * consecutive rows (2, 3, 4, 5 or 8); group q's distinct columns are
      * gword[gptr[q] .. gptr[q+1]-1], each (column << 8 | mask of the group's

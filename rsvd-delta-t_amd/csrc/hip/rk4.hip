@@ -1048,7 +1048,7 @@ extern "C" int rsvdt_hip_horner_stage_ex(const rsvdt_spmm_desc *d, const void *v
         return RSVDT_HIP_ERR_ARG;
     }
     hipStream_t st = as_stream(s);
-    if (d->variant == 9) return horner_tile(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
+    if (d->variant == 9 || d->variant == 10) return horner_tile(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
     if (d->prec == 64) {
         if (d->a_real) return dispatch_horner<double, true>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);
         return dispatch_horner<double, false>(d, vin, y, yscale, g, c, out, snap, snap_slot, snap_stride, st);

@@ -16,7 +16,9 @@
 // Lanes: KC lanes cover KC consecutive columns (one complex value each);
 // 256/KC row slots, RPT rows per slot, TR = RPT * 256/KC rows per tile.
 // When k > KC the tile is processed in column passes of KC columns, the
-// LDS staging repeated per pass.
+// LDS staging repeated per pass.  With PF (descriptor variant 10) the next
+// pass's first 6 rows per slot are loaded into registers while the current
+// pass computes from LDS, and written to LDS after its closing barrier.
 #include "common.h"
 
 namespace rsvdt {
@@ -59,7 +61,7 @@ __device__ __forceinline__ void tst(double2 *p, double2 v, bool nt) {
     else *p = v;
 }
 
-template <typename T, bool AREAL, int KC, int RPT, int UB>
+template <typename T, bool AREAL, int KC, int RPT, int UB, bool PF>
 __global__ __launch_bounds__(256) void k_horner_tile(
     int nrows, int k, int hmax, int emax, int xcd_remap, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ tptr, const int32_t *__restrict__ tcol,
@@ -106,6 +108,19 @@ __global__ __launch_bounds__(256) void k_horner_tile(
     // when y is also the gathered vector (first stage) its rows were just
     // staged through the caches: no nontemporal hint on it then
     const bool nty = (const void *)yv != (const void *)vin;
+    // PF: the first NS * UB distinct rows of the next pass are loaded into
+    // registers while the current pass computes from LDS (the rest, if any,
+    // one at a time after the barrier)
+    V xp[PF ? UB : 1];
+    if constexpr (PF) {
+        __syncthreads();                        // scol visible
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int i = rs + u * NS;
+            xp[u].x = 0; xp[u].y = 0;
+            if (i < h && lane < k) xp[u] = vb[(int64_t)scol[i] * k + lane];
+        }
+    }
 
     for (int j0 = 0; j0 < k; j0 += KC) {
         const int j = j0 + lane;
@@ -122,23 +137,42 @@ __global__ __launch_bounds__(256) void k_horner_tile(
                 if (g != nullptr) gr[q] = tld(g + idx, true);
             }
         }
-        __syncthreads();        // metadata visible; the previous pass's reads of sx are done
-        // stage the tile's distinct rows of vin, columns j0 .. j0+KC-1
-        for (int i0 = rs; i0 < h; i0 += NS * UB) {
-            V xb[UB];
+        if constexpr (PF) {
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
+                const int i = rs + u * NS;
+                if (i < h) sx[i * KC + lane] = xp[u];
+            }
+        } else {
+            __syncthreads();    // metadata visible; the previous pass's reads of sx are done
+        }
+        // stage the tile's distinct rows of vin, columns j0 .. j0+KC-1 (with
+        // PF only those beyond the first NS * UB, one load at a time)
+        constexpr int UL = PF ? 1 : UB;
+        for (int i0 = rs + (PF ? NS * UB : 0); i0 < h; i0 += NS * UL) {
+            V xb[UL];
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
                 const int i = i0 + u * NS;
                 xb[u].x = 0; xb[u].y = 0;
                 if (i < h && jok) xb[u] = vb[(int64_t)scol[i] * k + j];
             }
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
+            for (int u = 0; u < UL; ++u) {
                 const int i = i0 + u * NS;
                 if (i < h) sx[i * KC + lane] = xb[u];
             }
         }
         __syncthreads();
+        if constexpr (PF) {
+            const int jn = j0 + KC + lane;
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int i = rs + u * NS;
+                xp[u].x = 0; xp[u].y = 0;
+                if (i < h && jn < k) xp[u] = vb[(int64_t)scol[i] * k + jn];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int lr = rs + q * NS;
@@ -174,10 +208,11 @@ __global__ __launch_bounds__(256) void k_horner_tile(
             out[idx] = o;                       // gathered by the next stage: keep it cacheable
             if (snap != nullptr) tst(snap + idx, make_double2((double)o.x, (double)o.y), true);
         }
+        if constexpr (PF) __syncthreads();      // this pass's reads of sx are done
     }
 }
 
-template <typename T, bool AREAL, int KC, int RPT>
+template <typename T, bool AREAL, int KC, int RPT, bool PF>
 static int launch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g,
                        double c, void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
@@ -196,7 +231,7 @@ static int launch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *y,
     const unsigned gx = ceil_div(d->nrows, TR);
     const unsigned gy = (unsigned)d->nb;
     const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
-    hipLaunchKernelGGL((k_horner_tile<T, AREAL, KC, RPT, 8>), dim3(gx, gy), dim3(256), lay.total, s,
+    hipLaunchKernelGGL((k_horner_tile<T, AREAL, KC, RPT, PF ? 6 : 8, PF>), dim3(gx, gy), dim3(256), lay.total, s,
                        (int)d->nrows, (int)d->k, d->tile_hmax, d->tile_emax, remap, d->nrows * d->k, d->rowptr,
                        d->tptr, d->tcol, d->lidx, d->val, (const V *)vin, (const V *)y, (const double2 *)ys,
                        (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
@@ -211,8 +246,10 @@ static int dispatch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *
     int kc = d->lanes;
     if (kc == 0) kc = d->k <= 8 ? 8 : 16;
     const int rpt = d->tile_rows * kc / 256;
-#define RSVDT_TCASE(KC, RPT) \
-    if (kc == KC && rpt == RPT) return launch_tile<T, AREAL, KC, RPT>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
+#define RSVDT_TCASE(KC, RPT)                                                                                 \
+    if (kc == KC && rpt == RPT)                                                                              \
+        return d->variant == 10 ? launch_tile<T, AREAL, KC, RPT, true>(d, vin, y, ys, g, c, out, snap, slot, sstride, s) \
+                                : launch_tile<T, AREAL, KC, RPT, false>(d, vin, y, ys, g, c, out, snap, slot, sstride, s);
     RSVDT_TCASE(8, 1) RSVDT_TCASE(16, 1) RSVDT_TCASE(16, 2) RSVDT_TCASE(16, 3) RSVDT_TCASE(16, 4)
     RSVDT_TCASE(32, 1) RSVDT_TCASE(32, 2) RSVDT_TCASE(32, 4)
 #undef RSVDT_TCASE

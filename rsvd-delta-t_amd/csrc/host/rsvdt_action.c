@@ -131,7 +131,7 @@ static void desc_for(rsvdt_spmm_desc *d, Mat A, int dir, int prec, PetscInt k, P
         d->variant = 3;                                                         /* no column pairs */
     if (d->variant == 8 && (prec != 32 || g_rsvdt.opt.rk4_form || k % 4 || !(k / 4 == 4 || k / 4 == 8 || k / 4 == 16)))
         d->variant = 3;                                                         /* no double pairs */
-    if (d->variant == 9) {
+    if (d->variant == 9 || d->variant == 10) {
         if (g_rsvdt.opt.rk4_form || !tile_attach(d, A, dir, tile)) d->variant = 3;   /* no tiled form */
     }
 }
@@ -518,26 +518,29 @@ static PetscErrorCode autotune_lanes(Mat A, int dir, int prec, PetscInt k, void 
             if (ms < best) { best = ms; bl = (int)l; bv = v; br = r; }
         }
     }
-    /* the tiled kernel (variant 9, Horner form): columns per pass x tile
-     * height, on tiled forms built here on first use */
+    /* the tiled kernel (Horner form; variant 10 prefetches the next pass's
+     * rows into registers): columns per pass x tile height, on tiled forms
+     * built here on first use */
     int bt = 0;
     for (int ci = 0; ci < (int)(sizeof(tile_cand) / sizeof(tile_cand[0])) && !f; ci++) {
         const int kc = tile_cand[ci][0], tr = tile_cand[ci][1];
         if (kc > 8 && k <= kc / 2) continue;          /* more than half the lanes of a pass idle */
         rsvdt_spmm_desc dt = dsc;
         dt.lanes = kc;
-        dt.variant = 9;
         if (!tile_attach(&dt, A, dir, tr)) continue;
-        for (int r = 0; r < 4; r++) {
-            dt.xcd_remap = r;
-            if (rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream)) continue;
-            rsvdt_hip_event_record(e0, g_rsvdt.stream);
-            for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream);
-            rsvdt_hip_event_record(e1, g_rsvdt.stream);
-            float ms = 0;
-            rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
-            if (ms < best) { best = ms; bl = kc; bv = 9; br = r; bt = tr; }
-        }
+        for (int tv = 9; tv <= 10; tv++)
+            for (int r = 0; r < 4; r++) {
+                dt.variant = tv;
+                dt.xcd_remap = r;
+                if (rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream)) continue;
+                rsvdt_hip_event_record(e0, g_rsvdt.stream);
+                for (int rep = 0; rep < 3; rep++)
+                    (void)rsvdt_hip_horner_stage(&dt, a, b, NULL, NULL, 0.0, c, NULL, g_rsvdt.stream);
+                rsvdt_hip_event_record(e1, g_rsvdt.stream);
+                float ms = 0;
+                rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+                if (ms < best) { best = ms; bl = kc; bv = tv; br = r; bt = tr; }
+            }
     }
     rsvdt_hip_event_destroy(e0);
     rsvdt_hip_event_destroy(e1);

@@ -3,7 +3,8 @@ action, one power iteration, SVD before the adjoint, StoreU, adjoint action,
 final SVD) with transient removal on reduced grids of the benchmark's 2D and
 3D flow operators (real multi-field operators, 11 nonzeros per row), against
 the oracle at 1e-10 -- with the autotuned stage-kernel shape and with the
-tiled LDS-staged stage kernel forced -- and the discounting shift of
+tiled LDS-staged stage kernel forced (with and without the register
+prefetch of the next column pass) -- and the discounting shift of
 SourceCode/LNSStructure.c:53-55 (A <- A - beta I) through the library's
 LNSStructure and the rsvdt driver."""
 import ctypes
@@ -77,7 +78,7 @@ def mode_check(X, Xo, S, k, nwe, tol=1e-8):
 
 
 @pytest.mark.parametrize("name", ["flow2d", "flow3d"])
-@pytest.mark.parametrize("variant", [-1, 9])
+@pytest.mark.parametrize("variant", [-1, 9, 10])
 def test_flow_run_vs_oracle(pkg, name, variant):
     print(f"flow e2e {name} variant={variant}", flush=True)
     A, par, F, (Uo, Vo, So), nwe = case(name)
@@ -86,8 +87,8 @@ def test_flow_run_vs_oracle(pkg, name, variant):
     Ug, Vg, Sg = s.run(F)
     st = pkg.action_stats()
     s.close()
-    if variant == 9:
-        assert st["spmm_variant"] == 9
+    if variant >= 9:
+        assert st["spmm_variant"] == variant
     err = float(np.max(np.abs(Sg - So) / So))
     print(f"  max relative gain difference {err:.2e}, stage shape {st}", flush=True)
     assert err < 1e-10

@@ -8,7 +8,7 @@ import scipy.sparse as sp
 
 import kat
 from conftest import crandn, rel
-from gpu_util import Desc, Dev, grouped_form, init
+from gpu_util import Desc, Dev, grouped_form, init, tiled_form
 
 pytestmark = pytest.mark.gpu
 V = ctypes.c_void_p
@@ -222,6 +222,61 @@ def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced)
     snap = dsnap.get(np.complex128, y.shape)
     assert rel(out, ref) < (1e-14 if prec == 64 else 2e-6)
     assert np.array_equal(snap, out.astype(np.complex128))
+
+
+@pytest.mark.parametrize("variant", [9, 10])
+@pytest.mark.parametrize("k,real,prec,nb,kc,tr,remap,scaled,forced", [
+    (32, True, 64, 1, 8, 32, 3, False, True), (32, False, 64, 3, 8, 32, 0, True, True),
+    (16, True, 64, 2, 16, 16, 3, True, False), (12, False, 64, 1, 8, 32, 1, False, True),
+    (40, True, 64, 1, 16, 48, 2, True, True), (32, True, 32, 1, 16, 64, 3, False, True),
+    (16, False, 32, 2, 8, 32, 0, True, True), (32, True, 64, 1, 32, 8, 3, False, False),
+    (64, False, 64, 1, 32, 32, 0, True, True), (8, True, 64, 4, 8, 32, 0, True, True)])
+def test_horner_tile(L, pkg, k, real, prec, nb, kc, tr, remap, scaled, forced, variant):
+    """The Horner stage on the tiled form (variant 9; 10 with the next pass's
+    rows prefetched into registers): out_b = ys_b y_b + c A vin_b + g_b with
+    the snapshot copy, against numpy, on a banded multi-field-like operator
+    whose tiles share most of their columns (and k not a multiple of the
+    columns per pass)."""
+    print(f"horner tile variant={variant} k={k} real={real} prec={prec} nb={nb} kc={kc} tr={tr} remap={remap}",
+          flush=True)
+    rng = np.random.default_rng(11 * k + nb + variant)
+    N = 4001
+    offs = [-61, -7, -1, 0, 1, 2, 7, 61]
+    A = sp.diags([rng.standard_normal(N - abs(o)) for o in offs], offs, format="csr")
+    if not real:
+        A = A + 1j * sp.diags([rng.standard_normal(N - 3)], [3], format="csr")
+    A = A.tocsr().astype(np.complex128)
+    A.sort_indices()
+    c = 0.029
+    vin, y, g = crandn(rng, nb, N, k), crandn(rng, nb, N, k), crandn(rng, nb, N, k)
+    ys = crandn(rng, nb) if scaled else np.ones(nb, complex)
+    ref = np.stack([ys[b] * y[b] + c * (A @ vin[b]) + (g[b] if forced else 0) for b in range(nb)])
+    cdt = np.complex128 if prec == 64 else np.complex64
+    vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)
+    tptr, tcol, lidx, hmax, emax = tiled_form(A, tr)
+    drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
+    dtp, dtc, dli = Dev(pkg, tptr), Dev(pkg, tcol), Dev(pkg, lidx)
+    dvin, dy, dg = Dev(pkg, vin.astype(cdt)), Dev(pkg, y.astype(cdt)), Dev(pkg, g.astype(cdt))
+    dys = Dev(pkg, ys.astype(np.complex128))
+    dsnap = Dev(pkg, nbytes=y.astype(np.complex128).nbytes)
+    d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, kc, remap, variant, 0, None, None, None, tr, hmax, emax,
+             dtp.p, dtc.p, dli.p)
+    L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
+    # out aliases y (the in-place last stage of a step)
+    rc = L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, dys.p if scaled else None, dg.p if forced else None,
+                                  c, dy.p, dsnap.p, None)
+    assert rc == 0, L.rsvdt_hip_last_error()
+    out = dy.get(cdt, y.shape)
+    snap = dsnap.get(np.complex128, y.shape)
+    assert rel(out, ref) < (1e-14 if prec == 64 else 2e-6)
+    assert np.array_equal(snap, out.astype(np.complex128))
+    # the first stage gathers y itself (vin aliases y, no nontemporal hint on it)
+    dy2 = Dev(pkg, y.astype(cdt))
+    dout = Dev(pkg, nbytes=y.astype(cdt).nbytes)
+    rc = L.rsvdt_hip_horner_stage(ctypes.byref(d), dy2.p, dy2.p, None, None, c, dout.p, None, None)
+    assert rc == 0, L.rsvdt_hip_last_error()
+    ref1 = np.stack([y[b] + c * (A @ y[b]) for b in range(nb)])
+    assert rel(dout.get(cdt, y.shape), ref1) < (1e-14 if prec == 64 else 2e-6)
 
 
 @pytest.mark.parametrize("k,lanes", [(16, 16), (32, 32), (4, 4)])
