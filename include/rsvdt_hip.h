@@ -123,6 +123,10 @@ typedef struct {
     const int32_t *tptr;
     const int32_t *tcol;
     const uint16_t *lidx;
+    /* tiled form with nb > 1 blocks: each workgroup forms its tile in
+     * tile_bpw consecutive blocks (0 or 1: one block per workgroup), staging
+     * the tile's metadata once for all of them */
+    int tile_bpw;
 } rsvdt_spmm_desc;
 
 int rsvdt_hip_rk4_stage(const rsvdt_spmm_desc *d, int stage,

@@ -16,7 +16,9 @@
 // Lanes: KC lanes cover KC consecutive columns (one complex value each);
 // 256/KC row slots, RPT rows per slot, TR = RPT * 256/KC rows per tile.
 // When k > KC the tile is processed in column passes of KC columns, the
-// LDS staging repeated per pass.  With PF (descriptor variant 10) the next
+// LDS staging repeated per pass.  With tile_bpw > 1 a workgroup forms its
+// tile in tile_bpw consecutive blocks, one after the other, as further
+// passes over the same staged metadata.  With PF (descriptor variant 10) the next
 // pass's first 6 rows per slot are loaded into registers while the current
 // pass computes from LDS, and written to LDS after its closing barrier.
 #include "common.h"
@@ -63,7 +65,7 @@ __device__ __forceinline__ void tst(double2 *p, double2 v, bool nt) {
 
 template <typename T, bool AREAL, int KC, int RPT, int UB, bool PF>
 __global__ __launch_bounds__(256) void k_horner_tile(
-    int nrows, int k, int hmax, int emax, int xcd_remap, int64_t bstride,
+    int nrows, int k, int nb, int bpw, int hmax, int emax, int xcd_remap, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ tptr, const int32_t *__restrict__ tcol,
     const uint16_t *__restrict__ lidx, const void *__restrict__ vals_,
     const typename vec2<T>::type *__restrict__ vin, const typename vec2<T>::type *yv,
@@ -89,7 +91,11 @@ __global__ __launch_bounds__(256) void k_horner_tile(
     const int h0 = tptr[t], h = tptr[t + 1] - h0;
     const int e0 = rowptr[r0], ne = rowptr[r1] - e0;
     const int tid = threadIdx.x, lane = tid % KC, rs = tid / KC;
-    const int64_t boff = (int64_t)blockIdx.y * bstride;
+    // passes: bpw blocks of this workgroup x the column passes of a block,
+    // block-major; the tile's metadata is staged once for all of them
+    const int npc = (k + KC - 1) / KC;
+    const int b0 = blockIdx.y * bpw;
+    const int npass = (nb - b0 < bpw ? nb - b0 : bpw) * npc;
     const AV *vals = (const AV *)vals_;
     if (snap_slot != nullptr) {                 // snapshot slot read at run time (graph replays)
         const int sl = *snap_slot;
@@ -102,9 +108,6 @@ __global__ __launch_bounds__(256) void k_horner_tile(
         sli[i] = lidx[e0 + i];
     }
     for (int i = tid; i <= r1 - r0; i += 256) srp[i] = rowptr[r0 + i] - e0;
-    double ysr = 1.0, ysi = 0.0;
-    if (ys != nullptr) { ysr = ys[blockIdx.y].x; ysi = ys[blockIdx.y].y; }
-    const V *vb = vin + boff;
     // when y is also the gathered vector (first stage) its rows were just
     // staged through the caches: no nontemporal hint on it then
     const bool nty = (const void *)yv != (const void *)vin;
@@ -114,15 +117,21 @@ __global__ __launch_bounds__(256) void k_horner_tile(
     V xp[PF ? UB : 1];
     if constexpr (PF) {
         __syncthreads();                        // scol visible
+        const V *vb0 = vin + (int64_t)b0 * bstride;
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
             const int i = rs + u * NS;
             xp[u].x = 0; xp[u].y = 0;
-            if (i < h && lane < k) xp[u] = vb[(int64_t)scol[i] * k + lane];
+            if (i < h && lane < k) xp[u] = vb0[(int64_t)scol[i] * k + lane];
         }
     }
 
-    for (int j0 = 0; j0 < k; j0 += KC) {
+    for (int ps = 0; ps < npass; ++ps) {
+        const int bl = b0 + ps / npc, j0 = (ps % npc) * KC;
+        const int64_t boff = (int64_t)bl * bstride;
+        const V *vb = vin + boff;
+        double ysr = 1.0, ysi = 0.0;
+        if (ys != nullptr) { ysr = ys[bl].x; ysi = ys[bl].y; }
         const int j = j0 + lane;
         const bool jok = j < k;
         // row-local operands first, so that they are in flight during the staging
@@ -164,13 +173,16 @@ __global__ __launch_bounds__(256) void k_horner_tile(
             }
         }
         __syncthreads();
-        if constexpr (PF) {
-            const int jn = j0 + KC + lane;
+        if constexpr (PF) {                     // the next pass: next columns, or the next block
+            const int pn = ps + 1;
+            const int jn = (pn % npc) * KC + lane;
+            const bool more = pn < npass && jn < k;
+            const V *vbn = more ? vin + (int64_t)(b0 + pn / npc) * bstride : vin;
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 const int i = rs + u * NS;
                 xp[u].x = 0; xp[u].y = 0;
-                if (i < h && jn < k) xp[u] = vb[(int64_t)scol[i] * k + jn];
+                if (i < h && more) xp[u] = vbn[(int64_t)scol[i] * k + jn];
             }
         }
 #pragma unroll
@@ -228,11 +240,13 @@ static int launch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *y,
         set_error_msg("rsvdt_hip_horner_stage: a tile's columns do not fit in LDS (use fewer tile rows)");
         return RSVDT_HIP_ERR_ARG;
     }
+    const int bpw = d->tile_bpw > 1 ? d->tile_bpw : 1;
     const unsigned gx = ceil_div(d->nrows, TR);
-    const unsigned gy = (unsigned)d->nb;
+    const unsigned gy = ceil_div(d->nb, bpw);
     const int remap = (d->xcd_remap > 0 && d->xcd_remap <= 3 && gy == 1 && gx >= 8) ? d->xcd_remap : 0;
     hipLaunchKernelGGL((k_horner_tile<T, AREAL, KC, RPT, PF ? 6 : 8, PF>), dim3(gx, gy), dim3(256), lay.total, s,
-                       (int)d->nrows, (int)d->k, d->tile_hmax, d->tile_emax, remap, d->nrows * d->k, d->rowptr,
+                       (int)d->nrows, (int)d->k, (int)d->nb, bpw, d->tile_hmax, d->tile_emax, remap,
+                       d->nrows * d->k, d->rowptr,
                        d->tptr, d->tcol, d->lidx, d->val, (const V *)vin, (const V *)y, (const double2 *)ys,
                        (const V *)g, (V *)out, c, (double2 *)snap, slot, sstride);
     RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_stage(tiled)");
@@ -261,6 +275,10 @@ static int dispatch_tile(const rsvdt_spmm_desc *d, const void *vin, const void *
 int horner_tile(const rsvdt_spmm_desc *d, const void *vin, const void *y, const double *ys, const void *g, double c,
                 void *out, double *snap, const int *slot, int64_t sstride, hipStream_t s)
 {
+    if (d->tile_bpw < 0 || d->tile_bpw > 65535) {
+        set_error_msg("rsvdt_hip_horner_stage: tile_bpw must be 0..65535");
+        return RSVDT_HIP_ERR_ARG;
+    }
     if (!d->tptr || !d->tcol || !d->lidx || d->tile_rows <= 0 || d->tile_hmax <= 0 || d->tile_emax < 0 ||
         d->tile_hmax > 65536) {
         set_error_msg("rsvdt_hip_horner_stage: variant 9 needs the tiled operator arrays");

@@ -159,9 +159,10 @@ static void multi_block_shape(rsvdt_spmm_desc *d, PetscInt k)
 
 /* Shape of the FP64 Horner stage on nb > 1 blocks of k columns (A or A^H):
  * multi_block_shape's gather shape or the tiled kernel with 8 or 16 columns
- * per pass, whichever three launches on the caller's buffers (x gathered,
- * out written) time fastest; kept on the Mat for blocks of k columns.  An
- * explicit SpMMVariant or SpMMLanes option keeps multi_block_shape. */
+ * per pass and 1, 4 or 8 blocks per workgroup, whichever three launches on
+ * the caller's buffers (x gathered, out written) time fastest; kept on the
+ * Mat for blocks of k columns.  An explicit SpMMVariant or SpMMLanes option
+ * keeps multi_block_shape. */
 static void multi_block_tuned(rsvdt_spmm_desc *d, Mat A, int dir, PetscInt k, const void *x, void *out)
 {
     multi_block_shape(d, k);
@@ -175,36 +176,43 @@ static void multi_block_tuned(rsvdt_spmm_desc *d, Mat A, int dir, PetscInt k, co
             rsvdt_hip_event_destroy(e0);
             return;
         }
+        static const int bpw_cand[] = {1, 4, 8};
         float best = 1e30f;
-        int bl = d->lanes, bv = d->variant, bt = 0;
-        for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
-            rsvdt_spmm_desc t = *d;
-            if (cand[c][0]) {
-                if (cand[c][0] > 8 && k <= cand[c][0] / 2) continue;
-                t.lanes = cand[c][0];
-                t.variant = 9;
-                if (!tile_attach(&t, A, dir, cand[c][1])) continue;
+        int bl = d->lanes, bv = d->variant, bt = 0, bw = 0;
+        for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++)
+            for (int wi = 0; wi < (cand[c][0] ? 3 : 1); wi++) {
+                rsvdt_spmm_desc t = *d;
+                if (cand[c][0]) {
+                    if (cand[c][0] > 8 && k <= cand[c][0] / 2) continue;
+                    if (wi > 0 && bpw_cand[wi] > d->nb) continue;
+                    t.lanes = cand[c][0];
+                    t.variant = 9;
+                    t.tile_bpw = bpw_cand[wi];
+                    if (!tile_attach(&t, A, dir, cand[c][1])) continue;
+                }
+                if (rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream)) continue;
+                rsvdt_hip_event_record(e0, g_rsvdt.stream);
+                for (int rep = 0; rep < 3; rep++)
+                    (void)rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream);
+                rsvdt_hip_event_record(e1, g_rsvdt.stream);
+                float ms = 0;
+                rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
+                if (ms < best) { best = ms; bl = t.lanes; bv = t.variant; bt = cand[c][1]; bw = t.tile_bpw; }
             }
-            if (rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream)) continue;
-            rsvdt_hip_event_record(e0, g_rsvdt.stream);
-            for (int rep = 0; rep < 3; rep++) (void)rsvdt_hip_horner_stage(&t, x, x, NULL, NULL, 0.0, out, NULL, g_rsvdt.stream);
-            rsvdt_hip_event_record(e1, g_rsvdt.stream);
-            float ms = 0;
-            rsvdt_hip_event_elapsed_ms(e0, e1, &ms);
-            if (ms < best) { best = ms; bl = t.lanes; bv = t.variant; bt = cand[c][1]; }
-        }
         rsvdt_hip_event_destroy(e0);
         rsvdt_hip_event_destroy(e1);
         A->mt_ok[q] = 1; A->mt_k[q] = k; A->mt_lanes[q] = bl; A->mt_variant[q] = bv; A->mt_tile[q] = bt;
+        A->mt_bpw[q] = bw;
         if (g_rsvdt.opt.verbose)
-            rsvdt_printf("[rsvdt] autotune (%s, FP64, %lld blocks of %lld columns): lanes %d, variant %d, tile %d rows: "
-                         "%.1f us per stage\n", dir ? "A" : "A^H", (long long)d->nb, (long long)k, bl, bv, bt,
-                         1e3 * best / 3);
+            rsvdt_printf("[rsvdt] autotune (%s, FP64, %lld blocks of %lld columns): lanes %d, variant %d, tile %d rows, "
+                         "%d blocks per workgroup: %.1f us per stage\n", dir ? "A" : "A^H", (long long)d->nb,
+                         (long long)k, bl, bv, bt, bw, 1e3 * best / 3);
     }
     if (A->mt_variant[q] == 9) {
         rsvdt_spmm_desc t = *d;
         t.lanes = A->mt_lanes[q];
         t.variant = 9;
+        t.tile_bpw = A->mt_bpw[q];
         if (tile_attach(&t, A, dir, A->mt_tile[q])) *d = t;
     } else {
         d->lanes = A->mt_lanes[q];

@@ -72,8 +72,9 @@ struct _p_RSVDT_Mat {
     rsvdt_tiled tiled[2][RSVDT_NTILE];
     /* launch shape of the FP64 Horner stage on several blocks at once (the
      * e^{A dT} steps of the removal, the folded forcing), per [A, A^H], tuned
-     * for blocks of mt_k columns: lanes, variant, tile height */
-    int mt_ok[2], mt_lanes[2], mt_variant[2], mt_tile[2];
+     * for blocks of mt_k columns: lanes, variant, tile height, blocks per
+     * workgroup */
+    int mt_ok[2], mt_lanes[2], mt_variant[2], mt_tile[2], mt_bpw[2];
     PetscInt mt_k[2];
 };
 

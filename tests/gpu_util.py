@@ -49,7 +49,7 @@ class Desc(ctypes.Structure):
                 ("a_real", ctypes.c_int), ("prec", ctypes.c_int), ("lanes", ctypes.c_int), ("xcd_remap", ctypes.c_int),
                 ("variant", ctypes.c_int), ("group", ctypes.c_int), ("gptr", V_), ("gword", V_), ("gval", V_),
                 ("tile_rows", ctypes.c_int), ("tile_hmax", ctypes.c_int), ("tile_emax", ctypes.c_int),
-                ("tptr", V_), ("tcol", V_), ("lidx", V_)]
+                ("tptr", V_), ("tcol", V_), ("lidx", V_), ("tile_bpw", ctypes.c_int)]
 
 
 def tiled_form(A, TR):

@@ -225,20 +225,23 @@ def test_horner_stage(L, pkg, k, real, prec, nb, lanes, variant, scaled, forced)
 
 
 @pytest.mark.parametrize("variant", [9, 10])
-@pytest.mark.parametrize("k,real,prec,nb,kc,tr,remap,scaled,forced", [
-    (32, True, 64, 1, 8, 32, 3, False, True), (32, False, 64, 3, 8, 32, 0, True, True),
-    (16, True, 64, 2, 16, 16, 3, True, False), (12, False, 64, 1, 8, 32, 1, False, True),
-    (40, True, 64, 1, 16, 48, 2, True, True), (32, True, 32, 1, 16, 64, 3, False, True),
-    (16, False, 32, 2, 8, 32, 0, True, True), (32, True, 64, 1, 32, 8, 3, False, False),
-    (64, False, 64, 1, 32, 32, 0, True, True), (8, True, 64, 4, 8, 32, 0, True, True)])
-def test_horner_tile(L, pkg, k, real, prec, nb, kc, tr, remap, scaled, forced, variant):
+@pytest.mark.parametrize("k,real,prec,nb,kc,tr,remap,scaled,forced,bpw", [
+    (32, True, 64, 1, 8, 32, 3, False, True, 1), (32, False, 64, 3, 8, 32, 0, True, True, 1),
+    (16, True, 64, 2, 16, 16, 3, True, False, 1), (12, False, 64, 1, 8, 32, 1, False, True, 1),
+    (40, True, 64, 1, 16, 48, 2, True, True, 1), (32, True, 32, 1, 16, 64, 3, False, True, 1),
+    (16, False, 32, 2, 8, 32, 0, True, True, 1), (32, True, 64, 1, 32, 8, 3, False, False, 1),
+    (64, False, 64, 1, 32, 32, 0, True, True, 1), (8, True, 64, 4, 8, 32, 0, True, True, 1),
+    (8, True, 64, 7, 8, 32, 0, True, False, 3), (32, False, 64, 5, 16, 32, 3, True, True, 4),
+    (12, True, 32, 4, 8, 32, 0, False, True, 4)])
+def test_horner_tile(L, pkg, k, real, prec, nb, kc, tr, remap, scaled, forced, bpw, variant):
     """The Horner stage on the tiled form (variant 9; 10 with the next pass's
     rows prefetched into registers): out_b = ys_b y_b + c A vin_b + g_b with
     the snapshot copy, against numpy, on a banded multi-field-like operator
     whose tiles share most of their columns (and k not a multiple of the
-    columns per pass)."""
-    print(f"horner tile variant={variant} k={k} real={real} prec={prec} nb={nb} kc={kc} tr={tr} remap={remap}",
-          flush=True)
+    columns per pass), with one or several blocks per workgroup (the last
+    workgroup's share short when bpw does not divide nb)."""
+    print(f"horner tile variant={variant} k={k} real={real} prec={prec} nb={nb} kc={kc} tr={tr} remap={remap} "
+          f"bpw={bpw}", flush=True)
     rng = np.random.default_rng(11 * k + nb + variant)
     N = 4001
     offs = [-61, -7, -1, 0, 1, 2, 7, 61]
@@ -260,7 +263,7 @@ def test_horner_tile(L, pkg, k, real, prec, nb, kc, tr, remap, scaled, forced, v
     dys = Dev(pkg, ys.astype(np.complex128))
     dsnap = Dev(pkg, nbytes=y.astype(np.complex128).nbytes)
     d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, kc, remap, variant, 0, None, None, None, tr, hmax, emax,
-             dtp.p, dtc.p, dli.p)
+             dtp.p, dtc.p, dli.p, bpw)
     L.rsvdt_hip_horner_stage.argtypes = [ctypes.POINTER(Desc), V, V, V, V, ctypes.c_double, V, V, V]
     # out aliases y (the in-place last stage of a step)
     rc = L.rsvdt_hip_horner_stage(ctypes.byref(d), dvin.p, dy.p, dys.p if scaled else None, dg.p if forced else None,
