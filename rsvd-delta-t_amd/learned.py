@@ -448,7 +448,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=200)
     ap.add_argument("--eval", type=int, default=50)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--target-q", type=int, default=3,
+                    help="power iterations of the plain runs whose forcing modes are the training targets")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--save", default=None, help="write the trained weights here (.npz)")
     args = ap.parse_args()
@@ -458,7 +460,7 @@ def main():
     train_m = family_sample(rng, args.train)
     eval_m = family_sample(rng, args.eval)
     t0 = time.time()
-    data = training_data(pkg, train_m, log=log)
+    data = training_data(pkg, train_m, q=args.target_q, log=log)
     t_data = time.time() - t0
     net = SketchNet(seed=args.seed)
     t0 = time.time()
@@ -468,10 +470,12 @@ def main():
         net.save(args.save)
     if not eval_m:
         print(json.dumps({"learned_sketch": {"train_members": args.train, "epochs": args.epochs,
+                                             "target_q": args.target_q,
                                              "final_loss": hist[-1], "train_s": round(t_train, 1)}}))
         return
     rows, summary = evaluate(pkg, net, eval_m, qs=(0, 1, 2, 3), log=log)
     print(json.dumps({"learned_sketch": {"train_members": args.train, "eval_members": args.eval,
+                                         "target_q": args.target_q,
                                          "epochs": args.epochs, "final_loss": hist[-1],
                                          "data_s": round(t_data, 1), "train_s": round(t_train, 1),
                                          "summary": summary}}))
