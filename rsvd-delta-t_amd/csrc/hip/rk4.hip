@@ -823,13 +823,21 @@ __global__ __launch_bounds__(256) void k_horner_fused(
     for (int e = threadIdx.x; e < nnz; e += blockDim.x) { sv[e] = vals[e]; sci[e] = colidx[e]; }
     for (int r = threadIdx.x; r <= nrows; r += blockDim.x) srp[r] = rowptr[r];
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) sy[r] = y[boff + (int64_t)r * k + j];
+    // snapshot schedule by counters (as in k_horner_fused_reg below)
+    int64_t snap_st = nsteps, spos = 0;
+    if (snap != nullptr) {
+        const int64_t rel0 = i0 - Nt;
+        const int64_t first = rel0 >= 1 ? (rel0 + RR - 1) / RR * RR : RR;
+        snap_st = first - rel0;
+        spos = (first / RR - 1) % Nstore;
+    }
     __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
-        const int64_t i = i0 + st;
         int64_t pos = -1;
-        if (snap != nullptr && i > Nt && (i - Nt) % RR == 0) {
-            pos = ((i - Nt) / RR - 1) % Nstore;
-            if (!dir) pos = Nstore - 1 - pos;
+        if (st == snap_st) {
+            pos = dir ? spos : Nstore - 1 - spos;
+            spos = spos + 1 == Nstore ? 0 : spos + 1;
+            snap_st += RR;
         }
         const V *gs = g != nullptr ? g + (int64_t)st * gstride + boff : nullptr;
 #pragma unroll 1
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(256) void k_horner_fused(
 // The same with each thread's rows (at most RPT, 512 threads) and their
 // nonzeros (at most MAXNZ per row) held in registers for the whole launch:
 // a stage is then MAXNZ LDS reads and FMAs per row and one barrier.
-template <typename T, bool AREAL, int MAXNZ, int RPT>
+template <typename T, bool AREAL, int MAXNZ, int RPT, bool HASG>
 __global__ __launch_bounds__(512) void k_horner_fused_reg(
     int nrows, int k, int nsteps, int64_t bstride,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const void *__restrict__ vals_,
@@ -909,16 +917,37 @@ __global__ __launch_bounds__(512) void k_horner_fused_reg(
         }
         if (live[t]) sy[r] = y[boff + (int64_t)r * k + j];
     }
+    // Snapshot schedule, advanced by counters: step i = i0 + st with i > Nt and
+    // (i - Nt) % RR == 0 stores snapshot (i - Nt)/RR - 1 (mod Nstore), slot
+    // reversed for the adjoint.  The divisions run once, not once per step.
+    int64_t snap_st = nsteps, spos = 0;
+    if (snap != nullptr) {
+        const int64_t rel0 = i0 - Nt;
+        const int64_t first = rel0 >= 1 ? (rel0 + RR - 1) / RR * RR : RR;
+        snap_st = first - rel0;
+        spos = (first / RR - 1) % Nstore;
+    }
+    // The forcing term (HASG: g is not null) of step st + 1 is loaded as soon
+    // as step st's is used: its latency overlaps a whole step (barriers wait
+    // on LDS only).  The load is unconditional (the last step reloads its own
+    // sample) so that nothing has to wait for it before the next step.
+    V gcur[RPT];
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+        const int r = threadIdx.x + t * blockDim.x;
+        gcur[t] = V{};
+        if (HASG && live[t]) gcur[t] = g[boff + (int64_t)r * k + j];
+    }
     __syncthreads();
     for (int st = 0; st < nsteps; ++st) {
-        const int64_t i = i0 + st;
         int64_t pos = -1;
-        if (snap != nullptr && i > Nt && (i - Nt) % RR == 0) {
-            pos = ((i - Nt) / RR - 1) % Nstore;
-            if (!dir) pos = Nstore - 1 - pos;
+        if (st == snap_st) {
+            pos = dir ? spos : Nstore - 1 - spos;
+            spos = spos + 1 == Nstore ? 0 : spos + 1;
+            snap_st += RR;
         }
-        const V *gs = g != nullptr ? g + (int64_t)st * gstride + boff : nullptr;
-#pragma unroll 1
+        const V *gn = HASG ? g + (int64_t)(st + 1 < nsteps ? st + 1 : st) * gstride + boff : nullptr;
+#pragma unroll
         for (int stage = 0; stage < 4; ++stage) {
             const V *vin = stage == 0 ? sy : (stage == 2 ? sb : sa);
             V *out = stage == 3 ? sy : (stage == 1 ? sb : sa);
@@ -942,10 +971,10 @@ __global__ __launch_bounds__(512) void k_horner_fused_reg(
                 }
                 const V yv = sy[r];
                 double ox = (double)yv.x + cs * sr, oy = (double)yv.y + cs * si;
-                if (stage == 3 && gs != nullptr) {
-                    const V gv = gs[(int64_t)r * k + j];
-                    ox += (double)gv.x;
-                    oy += (double)gv.y;
+                if (HASG && stage == 3) {
+                    ox += (double)gcur[t].x;
+                    oy += (double)gcur[t].y;
+                    gcur[t] = gn[(int64_t)r * k + j];
                 }
                 V o;
                 o.x = (T)ox;
@@ -989,11 +1018,16 @@ extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, int
     hipStream_t st0 = as_stream(s);
     if (maxrow >= 0 && maxrow <= 4 && d->nrows <= 1024) {    // rows and nonzeros in registers
         const size_t lds3 = (size_t)3 * (size_t)d->nrows * (d->prec == 64 ? 16 : 8);
+#define RSVDT_FUSED_REG_G(T, AR, HG)                                                                              \
+        hipLaunchKernelGGL((k_horner_fused_reg<T, AR, 4, 2, HG>), grid, dim3(512), lds3, st0, (int)d->nrows,       \
+                           (int)d->k, (int)nsteps, d->nrows * d->k, d->rowptr, d->col, d->val,                      \
+                           (typename vec2<T>::type *)y, (const typename vec2<T>::type *)g, gstride, coef[0],       \
+                           coef[1], coef[2], coef[3], (double2 *)snap, snap_stride, i0, Nt, RR, Nstore, dir)
 #define RSVDT_FUSED_REG(T, AR)                                                                                    \
-        hipLaunchKernelGGL((k_horner_fused_reg<T, AR, 4, 2>), grid, dim3(512), lds3, st0, (int)d->nrows, (int)d->k,  \
-                           (int)nsteps, d->nrows * d->k, d->rowptr, d->col, d->val, (typename vec2<T>::type *)y,    \
-                           (const typename vec2<T>::type *)g, gstride, coef[0], coef[1], coef[2], coef[3],          \
-                           (double2 *)snap, snap_stride, i0, Nt, RR, Nstore, dir)
+        do {                                                                                                      \
+            if (g != nullptr) RSVDT_FUSED_REG_G(T, AR, true);                                                     \
+            else RSVDT_FUSED_REG_G(T, AR, false);                                                                 \
+        } while (0)
         if (d->prec == 64) {
             if (d->a_real) RSVDT_FUSED_REG(double, true);
             else RSVDT_FUSED_REG(double, false);
@@ -1002,6 +1036,7 @@ extern "C" int rsvdt_hip_horner_fused(const rsvdt_spmm_desc *d, int64_t nnz, int
             else RSVDT_FUSED_REG(float, false);
         }
 #undef RSVDT_FUSED_REG
+#undef RSVDT_FUSED_REG_G
         RSVDT_LAUNCH_CHECK("rsvdt_hip_horner_fused");
         return RSVDT_HIP_OK;
     }

@@ -257,20 +257,41 @@ PetscErrorCode CreateDFTiDFTMats(RSVDt_vars *RSVDt, DFT_matrices *D, LNS_vars *L
     if (Nwe <= 0 || Nt2 <= 0) return rsvdt_err(RSVDT_ERR_USER, "CreateDFTiDFTMats", "grid not set up");
     double *idft = malloc(sizeof(double) * 2 * (size_t)(Nwe * Nt2));
     double *dft = malloc(sizeof(double) * 2 * (size_t)(Nw * Nwe));
-    if (!idft || !dft) return rsvdt_err(RSVDT_ERR_MEM, "CreateDFTiDFTMats", "out of memory");
+    double *tw = malloc(sizeof(double) * 2 * (size_t)Nt2);
+    PetscInt *mal = malloc(sizeof(PetscInt) * 2 * (size_t)Nwe), *pos = mal + Nwe;
+    if (!idft || !dft || !tw || !mal) {
+        free(idft); free(dft); free(tw); free(mal);
+        return rsvdt_err(RSVDT_ERR_MEM, "CreateDFTiDFTMats", "out of memory");
+    }
+    /* exp(alpha*m_alias*j) depends only on (m_alias*j) mod 2Ns: one table of
+     * the 2Ns roots of unity (conjugated and scaled by 2/(2Ns)), walked with an
+     * integer phase per frequency, in place of a cos and a sin per entry.  The
+     * argument reduction is exact, so the entries equal the direct form's to
+     * rounding; the idft columns are written contiguously. */
     const double ai = -2.0 * PI_D / (double)Nt2;             /* alpha = i*ai */
+    for (PetscInt m = 0; m < Nt2; m++) {
+        const double ph = ai * (double)m;
+        tw[2 * m] = cos(ph) * (2.0 / (double)Nt2);
+        tw[2 * m + 1] = -sin(ph) * (2.0 / (double)Nt2);       /* conj */
+    }
     for (PetscInt iw = 0; iw < Nwe; iw++) {
-        double mal;
-        if (RSVDt->TS.RealOperator) mal = (double)iw;
-        else if (Nwe % 2 == 0) mal = (iw <= Nwe / 2 - 1) ? (double)iw : (double)((iw - Nwe) + Nt2);
-        else mal = (iw <= (Nwe - 1) / 2) ? (double)iw : (double)((iw - Nwe) + Nt2);
-        const double vi = ai * mal;                           /* v = i*vi */
-        for (PetscInt j = 0; j < Nt2; j++) {
-            const double ph = vi * (double)j;                 /* exp(v*j) = cos(ph) + i sin(ph) */
-            idft[2 * ((size_t)j * Nwe + iw)] = cos(ph) * (2.0 / (double)Nt2);
-            idft[2 * ((size_t)j * Nwe + iw) + 1] = -sin(ph) * (2.0 / (double)Nt2);   /* conj */
+        PetscInt a;
+        if (RSVDt->TS.RealOperator) a = iw;
+        else if (Nwe % 2 == 0) a = (iw <= Nwe / 2 - 1) ? iw : (iw - Nwe) + Nt2;
+        else a = (iw <= (Nwe - 1) / 2) ? iw : (iw - Nwe) + Nt2;
+        mal[iw] = ((a % Nt2) + Nt2) % Nt2;
+        pos[iw] = 0;
+    }
+    for (PetscInt j = 0; j < Nt2; j++) {
+        double *col = idft + 2 * (size_t)j * (size_t)Nwe;
+        for (PetscInt iw = 0; iw < Nwe; iw++) {
+            col[2 * iw] = tw[2 * pos[iw]];
+            col[2 * iw + 1] = tw[2 * pos[iw] + 1];
+            pos[iw] += mal[iw];
+            if (pos[iw] >= Nt2) pos[iw] -= Nt2;
         }
     }
+    free(tw); free(mal);
     const double a2 = -2.0 * PI_D / (RSVDt->TS.RealOperator ? (double)(2 * Nwe) : (double)Nwe);
     for (PetscInt iw = 0; iw < Nwe; iw++)
         for (PetscInt s = 0; s < Nw; s++) {

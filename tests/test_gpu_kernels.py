@@ -331,29 +331,13 @@ def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps, band):
     A = (A - 2 * sp.eye(N)).tocsr().astype(np.complex128)
     A.sort_indices()
     h, nsteps = 0.05, 23
-    i0, Nt, RR, Nstore, dr = 5, 9, 4, 3, 0
+    Nt, RR, Nstore = 9, 4, 3
     y0 = crandn(rng, nb, N, k)
     g = crandn(rng, nsteps, N, k) if nb == 1 else None
     cdt = np.complex128 if prec == 64 else np.complex64
-    y = y0.astype(cdt).astype(np.complex128)
-    snap_ref = np.zeros((Nstore, N, k), complex)
-    for st in range(nsteps):
-        for b in range(nb):
-            v = y[b]
-            for c in (h / 4, h / 3, h / 2):
-                v = (y[b] + c * (A @ v)).astype(cdt).astype(np.complex128) if prec == 32 else y[b] + c * (A @ v)
-            yn = y[b] + h * (A @ v) + (g[st].astype(cdt) if g is not None else 0)
-            y[b] = yn.astype(cdt).astype(np.complex128)
-        i = i0 + st
-        if snaps and i > Nt and (i - Nt) % RR == 0:
-            pos = ((i - Nt) // RR - 1) % Nstore
-            pos = Nstore - 1 - pos if not dr else pos
-            snap_ref[pos] = y[0]
     vals = A.data.real.astype(np.float64) if real else A.data.astype(np.complex128)
     drp = Dev(pkg, A.indptr.astype(np.int32)); dci = Dev(pkg, A.indices.astype(np.int32)); dv = Dev(pkg, vals)
-    dy = Dev(pkg, y0.astype(cdt))
     dg = Dev(pkg, g.astype(cdt)) if g is not None else None
-    dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
     coef = np.array([h / 4, h / 3, h / 2, h])
     d = Desc(N, k, nb, drp.p, dci.p, dv.p, int(real), prec, 0, 0, 0)
     L.rsvdt_hip_horner_fused.argtypes = [ctypes.POINTER(Desc), I64, I64, V, V, I64, V, I64, V, I64, I64, I64, I64,
@@ -362,18 +346,36 @@ def test_horner_fused(L, pkg, N, k, real, prec, nb, snaps, band):
     assert L.rsvdt_hip_horner_fused_fits(N, A.nnz, int(real), prec) == 1
     maxrow = int(np.max(np.diff(A.indptr)))
     tol = 1e-12 if prec == 64 else 3e-5
-    # maxrow -1: nonzeros staged in LDS; the true maxrow: in registers when the rows are short enough
-    for mr in (-1, maxrow):
-        dy = Dev(pkg, y0.astype(cdt))
-        dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
-        rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), A.nnz, mr, dy.p, dg.p if dg else None, N * k,
-                                      coef.ctypes.data_as(V), nsteps, dsnap.p if snaps else None, N * k, i0, Nt, RR,
-                                      Nstore, dr, None)
-        assert rc == 0, L.rsvdt_hip_last_error()
-        out = dy.get(cdt, y.shape)
-        assert rel(out, y) < tol, mr
-        if snaps:
-            assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol
+    # two snapshot schedules: the run starts before the first snapshot (direct
+    # slot order), and inside the snapshot period off the snapshot grid
+    # (adjoint slot order, reversed); both wrap around the Nstore slots
+    for i0, dr in ((5, 0), (15, 1)):
+        y = y0.astype(cdt).astype(np.complex128)
+        snap_ref = np.zeros((Nstore, N, k), complex)
+        for st in range(nsteps):
+            for b in range(nb):
+                v = y[b]
+                for c in (h / 4, h / 3, h / 2):
+                    v = (y[b] + c * (A @ v)).astype(cdt).astype(np.complex128) if prec == 32 else y[b] + c * (A @ v)
+                yn = y[b] + h * (A @ v) + (g[st].astype(cdt) if g is not None else 0)
+                y[b] = yn.astype(cdt).astype(np.complex128)
+            i = i0 + st
+            if snaps and i > Nt and (i - Nt) % RR == 0:
+                pos = ((i - Nt) // RR - 1) % Nstore
+                pos = Nstore - 1 - pos if not dr else pos
+                snap_ref[pos] = y[0]
+        # maxrow -1: nonzeros staged in LDS; the true maxrow: in registers when the rows are short enough
+        for mr in (-1, maxrow):
+            dy = Dev(pkg, y0.astype(cdt))
+            dsnap = Dev(pkg, np.zeros((Nstore, N, k), complex))
+            rc = L.rsvdt_hip_horner_fused(ctypes.byref(d), A.nnz, mr, dy.p, dg.p if dg else None, N * k,
+                                          coef.ctypes.data_as(V), nsteps, dsnap.p if snaps else None, N * k, i0, Nt,
+                                          RR, Nstore, dr, None)
+            assert rc == 0, L.rsvdt_hip_last_error()
+            out = dy.get(cdt, y.shape)
+            assert rel(out, y) < tol, (mr, i0)
+            if snaps:
+                assert rel(dsnap.get(np.complex128, snap_ref.shape), snap_ref) < tol, (mr, i0)
     # an operator too large for LDS is refused before any launch
     nbig = 4000
     assert L.rsvdt_hip_horner_fused_fits(nbig, 10 * nbig, int(real), prec) == 0
