@@ -39,8 +39,14 @@ typedef void *rsvdt_stream_t;        /* hipStream_t, or NULL for the null stream
 /* ---- runtime ---------------------------------------------------------- */
 int rsvdt_hip_init(int device);                       /* select device, check gfx950 */
 const char *rsvdt_hip_last_error(void);               /* text of the last failure */
+/* device memory; freed blocks of at most 256 MB are kept (2 GB in all) for
+ * later requests of their size up to 2x, after the device wait hipFree itself
+ * makes; release_cached frees them (RSVDT_Finalize calls it); cached_bytes
+ * is what the cache holds, which mem_info counts as free */
 int rsvdt_hip_malloc(void **p, size_t bytes);
 int rsvdt_hip_free(void *p);
+int rsvdt_hip_release_cached(void);
+size_t rsvdt_hip_cached_bytes(void);
 /* synchronous; both wait for all queued device work first */
 int rsvdt_hip_memcpy_h2d(void *dst, const void *src_host, size_t bytes);
 int rsvdt_hip_memcpy_d2h(void *dst_host, const void *src, size_t bytes);

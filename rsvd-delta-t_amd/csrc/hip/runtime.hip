@@ -2,6 +2,8 @@
 // small elementwise kernels (fills, conversions, strided copies, norms).
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
+#include <unordered_map>
 #include "common.h"
 
 namespace rsvdt {
@@ -41,15 +43,94 @@ extern "C" int rsvdt_hip_init(int device) {
     return rsvdt_noalloc_setup(device);     // device address of the no-allocation entry point's scratch
 }
 
+// Device allocations keep a small cache of freed blocks.  hipFree of a block
+// costs 0.3-0.5 ms (the unmapping; it also waits for the device), and an
+// action frees a dozen work blocks, which on small operators is as long as
+// its time stepping.  A freed block of at most 256 MB is kept instead, after
+// the same wait for the device that hipFree performs (so no queued work on any
+// stream can still use it), and handed to a later request of its size up to
+// 2x.  At most 2 GB (64 blocks) is held; a failing hipMalloc empties the cache
+// and tries again; rsvdt_hip_release_cached frees it all (RSVDT_Finalize
+// calls it), and rsvdt_hip_mem_info counts it as free.
+namespace {
+struct cached_blk {
+    void *p;
+    size_t bytes;
+};
+constexpr size_t kCacheMaxBlock = (size_t)256 << 20;
+constexpr size_t kCacheMaxTotal = (size_t)2 << 30;
+constexpr int kCacheSlots = 64;
+std::mutex g_alloc_mu;
+cached_blk g_cache[kCacheSlots];
+int g_ncache = 0;
+size_t g_cache_bytes = 0;
+std::unordered_map<void *, size_t> g_live;     // this allocator's blocks and their rounded sizes
+
+size_t round_alloc(size_t b) { return (b + 65535) & ~(size_t)65535; }
+
+// with g_alloc_mu held
+void cache_release_locked() {
+    for (int i = 0; i < g_ncache; ++i) (void)hipFree(g_cache[i].p);
+    g_ncache = 0;
+    g_cache_bytes = 0;
+}
+}  // namespace
+
 extern "C" int rsvdt_hip_malloc(void **p, size_t bytes) {
     *p = nullptr;
-    if (bytes == 0) bytes = 256;
-    RSVDT_HIP_TRY(hipMalloc(p, bytes));
+    const size_t r = round_alloc(bytes ? bytes : 256);
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (r <= kCacheMaxBlock) {
+        int best = -1;
+        for (int i = 0; i < g_ncache; ++i)
+            if (g_cache[i].bytes >= r && g_cache[i].bytes <= 2 * r && (best < 0 || g_cache[i].bytes < g_cache[best].bytes))
+                best = i;
+        if (best >= 0) {
+            *p = g_cache[best].p;
+            g_live[*p] = g_cache[best].bytes;
+            g_cache_bytes -= g_cache[best].bytes;
+            g_cache[best] = g_cache[--g_ncache];
+            return RSVDT_HIP_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, r);
+    if (e != hipSuccess && g_ncache > 0) {
+        (void)hipGetLastError();
+        cache_release_locked();
+        e = hipMalloc(p, r);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc", e);
+        return RSVDT_HIP_ERR_RUNTIME;
+    }
+    g_live[*p] = r;
     return RSVDT_HIP_OK;
 }
 extern "C" int rsvdt_hip_free(void *p) {
-    if (p) RSVDT_HIP_TRY(hipFree(p));
+    if (!p) return RSVDT_HIP_OK;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_live.find(p);
+    const size_t sz = it != g_live.end() ? it->second : 0;
+    if (it != g_live.end()) g_live.erase(it);
+    if (sz && sz <= kCacheMaxBlock && g_ncache < kCacheSlots && g_cache_bytes + sz <= kCacheMaxTotal) {
+        RSVDT_HIP_TRY(hipDeviceSynchronize());
+        g_cache[g_ncache++] = cached_blk{p, sz};
+        g_cache_bytes += sz;
+        return RSVDT_HIP_OK;
+    }
+    RSVDT_HIP_TRY(hipFree(p));
     return RSVDT_HIP_OK;
+}
+extern "C" int rsvdt_hip_release_cached(void) {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    RSVDT_HIP_TRY(hipDeviceSynchronize());
+    cache_release_locked();
+    return RSVDT_HIP_OK;
+}
+extern "C" size_t rsvdt_hip_cached_bytes(void) {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    return g_cache_bytes;
 }
 // Host<->device copies first wait for the whole device: the engine's work runs
 // on non-blocking streams, which the null-stream hipMemcpy does not order with.
@@ -73,6 +154,9 @@ extern "C" int rsvdt_hip_sync(void) {
 }
 extern "C" int rsvdt_hip_mem_info(size_t *f, size_t *t) {
     RSVDT_HIP_TRY(hipMemGetInfo(f, t));
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    *f += g_cache_bytes;                   // cached blocks are given back on demand
+    if (*f > *t) *f = *t;
     return RSVDT_HIP_OK;
 }
 extern "C" int rsvdt_hip_stream_create(rsvdt_stream_t *s) {

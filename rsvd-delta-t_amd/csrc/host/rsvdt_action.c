@@ -162,12 +162,15 @@ static void multi_block_shape(rsvdt_spmm_desc *d, PetscInt k)
  * per pass and 1, 4 or 8 blocks per workgroup, whichever three launches on
  * the caller's buffers (x gathered, out written) time fastest; kept on the
  * Mat for blocks of k columns.  An explicit SpMMVariant or SpMMLanes option
- * keeps multi_block_shape. */
+ * keeps multi_block_shape, and so do blocks of less than 32 MB in all: a
+ * launch there takes about 10 us, and the ~40 timed candidate launches (and
+ * building the tiled forms) would cost more than any shape saves. */
 static void multi_block_tuned(rsvdt_spmm_desc *d, Mat A, int dir, PetscInt k, const void *x, void *out)
 {
     multi_block_shape(d, k);
     if (g_rsvdt.opt.rk4_form || g_rsvdt.opt.spmm_variant >= 0 || g_rsvdt.opt.spmm_lanes || !g_rsvdt.opt.autotune)
         return;
+    if ((double)d->nrows * (double)k * (double)d->nb * 16.0 < 32e6) return;
     const int q = dir ? 0 : 1;
     if (!(A->mt_ok[q] && A->mt_k[q] == k)) {
         static const int cand[][2] = {{0, 0}, {8, 32}, {16, 16}, {16, 32}};
@@ -221,13 +224,16 @@ static void multi_block_tuned(rsvdt_spmm_desc *d, Mat A, int dir, PetscInt k, co
 }
 
 /* Forcing samples per chunk of S steps: S (form 0) or 2S+1 (form 1).  The
- * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; S shrinks
- * while the two chunk buffers would take more than 60 % of free memory (an
- * action frees large ones when its stepping ends, before the transient
- * removal's work). */
+ * defaults, 64 and 2*31+1 = 63, match the GEMM's 64-row MFMA tile; with the
+ * LDS-resident stepper (fused: one launch per chunk of a small state) the
+ * chunk starts at 256 steps, since each chunk also costs the main stream a
+ * wait on the forcing stream (~20 us, a quarter of a 64-step launch there).
+ * S shrinks while the two chunk buffers would take more than 60 % of free
+ * memory (an action frees large ones when its stepping ends, before the
+ * transient removal's work). */
 static PetscInt chunk_samples(int form, PetscInt S) { return form ? 2 * S + 1 : S; }
 
-static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
+static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz, int fused)
 {
     if (g_rsvdt.opt.chunk_steps > 0) return g_rsvdt.opt.chunk_steps;
     size_t fr = 0, tot = 0;
@@ -235,13 +241,13 @@ static PetscInt auto_chunk(int form, PetscInt N, PetscInt k, size_t esz)
     /* the chunk buffers of an earlier action are reused (or replaced) */
     fr += rsvdt_scratch_size(SCR_FBUF) + rsvdt_scratch_size(SCR_FBUF2);
     double blk = (double)N * (double)k * (double)esz;
-    PetscInt S = form ? 31 : 64;
+    PetscInt S = form ? 31 : (fused ? 256 : 64);
     while (S > 4 && 2.0 * (double)chunk_samples(form, S) * blk > 0.6 * (double)fr) S = form ? (S - 1) / 2 : S / 2;
     return S;
 }
 
 static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double h, PetscInt k, PetscInt Nwe,
-                                   PetscInt Ns, const double *Fin, const double *idft)
+                                   PetscInt Ns, const double *Fin, const double *idft, int fused)
 {
     memset(st, 0, sizeof(*st));
     st->A = A; st->dir = dir; st->prec = prec; st->h = h;
@@ -249,7 +255,7 @@ static PetscErrorCode stepper_init(stepper *st, Mat A, int dir, int prec, double
     st->N = A->rows; st->k = k; st->Nwe = Nwe; st->twoNs = 2 * Ns;
     st->esz = prec == 64 ? 16 : 8;
     st->Fin = Fin; st->idft = idft;
-    st->S = auto_chunk(st->form, st->N, k, st->esz);
+    st->S = auto_chunk(st->form, st->N, k, st->esz, fused);
     desc_for(&st->desc, A, dir, prec, k, 1);
     size_t blk = (size_t)st->N * (size_t)k * st->esz;
     const size_t nsamp = (size_t)chunk_samples(st->form, st->S);
@@ -688,10 +694,11 @@ static PetscErrorCode time_stepping(Mat A, DFT_matrices *D, RSVDt_vars *V, Mat F
     memset(&st, 0, sizeof(st));
     for (int seg = 0; seg < (split ? 2 : 1) && !ierr; seg++) {
         const int p = seg_prec[seg];
-        ierr = stepper_init(&st, A, dir, p, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d);
-        if (ierr) break;
         /* operators whose state fits in LDS: whole chunks of Horner steps per launch */
-        const int fused = st.form == 0 && g_rsvdt.opt.fuse_small && rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, p);
+        const int fused = !g_rsvdt.opt.rk4_form && g_rsvdt.opt.fuse_small &&
+                          rsvdt_hip_horner_fused_fits(N, A->nnz, A->is_real, p);
+        ierr = stepper_init(&st, A, dir, p, V->TS.dt, k, Nwe, V->TS.Ns, Fhat->d, D->idft->d, fused);
+        if (ierr) break;
         if (g_rsvdt.opt.autotune && !g_rsvdt.opt.spmm_lanes && !fused) {
             ierr = autotune_lanes(A, dir, p, k, st.u1, st.y, st.u2, st.form ? st.acc : st.u2);
             desc_for(&st.desc, A, dir, p, k, 1);

@@ -250,6 +250,7 @@ PetscErrorCode RSVDT_Finalize(void)
     if (g_rsvdt.device >= 0) rsvdt_action_release();
     rsvdt_scratch_free_all();
     if (g_rsvdt.device >= 0) {
+        rsvdt_hip_release_cached();
         rsvdt_hip_stream_destroy(g_rsvdt.stream);
         rsvdt_hip_stream_destroy(g_rsvdt.stream_frc);
         g_rsvdt.stream = g_rsvdt.stream_frc = NULL;

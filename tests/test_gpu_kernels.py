@@ -484,3 +484,55 @@ def test_transpose_rows(L, pkg, n, N, kt, b0, kb):
     B = dB.get(np.complex128, Y.shape)
     assert np.array_equal(B[:, :, b0:b0 + kb], Y[:, :, b0:b0 + kb])
     assert not np.any(np.delete(B, np.s_[b0:b0 + kb], axis=2))
+
+
+def test_allocation_cache(L, pkg):
+    """Freed device blocks of at most 256 MB are kept and handed to a later
+    request of their size (up to 2x); larger ones are freed; the cache counts
+    as free memory and release_cached empties it.  Garbage collection is held
+    off so that no earlier test's buffer is freed into the cache meanwhile."""
+    import gc
+    gc.collect()
+    gc.disable()
+    try:
+        _allocation_cache_checks(L)
+    finally:
+        gc.enable()
+
+
+def _allocation_cache_checks(L):
+    V = ctypes.c_void_p
+    L.rsvdt_hip_malloc.argtypes = [ctypes.POINTER(V), ctypes.c_size_t]
+    L.rsvdt_hip_free.argtypes = [V]
+    L.rsvdt_hip_cached_bytes.restype = ctypes.c_size_t
+    L.rsvdt_hip_mem_info.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    assert L.rsvdt_hip_release_cached() == 0
+    assert L.rsvdt_hip_cached_bytes() == 0
+    p = V()
+    assert L.rsvdt_hip_malloc(ctypes.byref(p), 3 << 20) == 0
+    first = p.value
+    # the block is usable: a device round trip through it
+    x = np.arange(1024, dtype=np.float64)
+    assert L.rsvdt_hip_memcpy_h2d(p, x.ctypes.data_as(V), ctypes.c_size_t(x.nbytes)) == 0
+    assert L.rsvdt_hip_free(p) == 0
+    assert L.rsvdt_hip_cached_bytes() == 3 << 20
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    assert L.rsvdt_hip_mem_info(ctypes.byref(f), ctypes.byref(t)) == 0 and f.value <= t.value
+    q = V()
+    assert L.rsvdt_hip_malloc(ctypes.byref(q), (2 << 20) + 5) == 0      # within 2x: the cached block
+    assert q.value == first and L.rsvdt_hip_cached_bytes() == 0
+    y = np.zeros(1024)
+    assert L.rsvdt_hip_memcpy_h2d(q, x.ctypes.data_as(V), ctypes.c_size_t(x.nbytes)) == 0
+    assert L.rsvdt_hip_memcpy_d2h(y.ctypes.data_as(V), q, ctypes.c_size_t(y.nbytes)) == 0
+    assert np.array_equal(x, y)
+    assert L.rsvdt_hip_free(q) == 0
+    r = V()
+    assert L.rsvdt_hip_malloc(ctypes.byref(r), 1 << 20) == 0            # 3 MB is over 2x of 1 MB: a new block
+    assert r.value != first and L.rsvdt_hip_cached_bytes() == 3 << 20
+    assert L.rsvdt_hip_free(r) == 0
+    big = V()
+    assert L.rsvdt_hip_malloc(ctypes.byref(big), 300 << 20) == 0
+    assert L.rsvdt_hip_free(big) == 0
+    assert L.rsvdt_hip_cached_bytes() == (3 << 20) + (1 << 20)            # the 300 MB block was freed
+    assert L.rsvdt_hip_release_cached() == 0
+    assert L.rsvdt_hip_cached_bytes() == 0
