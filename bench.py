@@ -489,7 +489,8 @@ def time_to_modes_gl(pkg, nmembers):
         return {"skipped": f"no trained weights at {os.path.relpath(learned.WEIGHTS, ROOT)}"}
     net = learned.SketchNet(seed=0).load(learned.WEIGHTS)
     members = learned.family_sample(np.random.default_rng(777), nmembers)
-    rows, summ = learned.time_to_modes(pkg, net, members)
+    rows, summ = learned.time_to_modes(pkg, net, members,
+                                       log=lambda s: print(f"[ttm] {s}", file=sys.stderr, flush=True))
     summ = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
     summ["speedup_vs_best_random_fp64"] = round(summ["fast_random"]["s"] / summ["fast_s"], 3)
     summ["members_evaluated"] = len(rows)

@@ -403,7 +403,7 @@ FAST32 = dict(precision=32, rk4_form=0, fuse_small=1)
 
 
 def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242, fast=FAST, plain=PLAIN,
-                  extra=(("fast_fp32", FAST32, True), ("fast_random", FAST, False))):
+                  extra=(("fast_fp32", FAST32, True), ("fast_random", FAST, False)), log=None):
     """Wall time to sigma_1 relative error <= tol (against a converged
     reference: q_ref power iterations, optimised form) for the plain form
     with a random sketch and for the optimised form with the learned sketch,
@@ -411,7 +411,8 @@ def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242,
     (name, options, learned columns or not): by default the optimised form
     with the FP32 state, and the optimised FP64 form with a random sketch --
     the engine's best form without the network, which isolates what the
-    learned columns buy.  Returns per-member rows and the mean times."""
+    learned columns buy.  log: optional callable given one line per
+    member.  Returns per-member rows and the mean times."""
     N = FAMILY["N"]
     x = np.linspace(-85, 85, N)
     rows = []
@@ -431,6 +432,9 @@ def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242,
                     row[name] = {"q": q, "wall_s": wall, "sigma1_err": err}
                     break
         rows.append(row)
+        if log:
+            log(f"member {i + 1}/{len(members)}: " + ", ".join(
+                f"{n} q={row[n]['q']} {1e3 * row[n]['wall_s']:.1f} ms" for n in row if isinstance(row[n], dict)))
     ok = all(r["plain"]["sigma1_err"] <= tol and r["fast"]["sigma1_err"] <= tol for r in rows)
     tp = float(np.mean([r["plain"]["wall_s"] for r in rows]))
     tf = float(np.mean([r["fast"]["wall_s"] for r in rows]))
