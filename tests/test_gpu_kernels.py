@@ -79,10 +79,12 @@ def test_bgram_bapply(L, pkg, nb, nr, na, nc, layout):
     assert rel(out, ref) < 1e-13
 
 
-@pytest.mark.parametrize("nb,n", [(5, 16), (3, 65), (2, 70), (2, 96), (2, 129), (8, 1)])
+@pytest.mark.parametrize("nb,n", [(5, 16), (3, 20), (3, 33), (2, 40), (3, 65), (2, 70), (2, 81), (2, 96), (2, 129),
+                                  (8, 1)])
 def test_chol_jacobi_solve(L, pkg, nb, n):
-    """Cholesky + inverse (LDS-resident for n <= 70, global beyond), Jacobi SVD,
-    shifted LU solves (LDS-resident for n <= 96, global beyond) against numpy."""
+    """Cholesky + inverse (in registers for n <= 80, global memory beyond),
+    Jacobi SVD, shifted solves (registers for n <= 79, LDS for n <= 96,
+    global beyond) against numpy."""
     print(f"dense nb={nb} n={n}", flush=True)
     rng = np.random.default_rng(n)
     X = crandn(rng, nb, 4 * n + 3, n)
