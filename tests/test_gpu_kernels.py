@@ -22,8 +22,11 @@ def L(pkg):
 
 @pytest.mark.parametrize("P,K,Lc,p0,sgn,cmod,oprec", [(64, 64, 5000, 3, 1, 200, 64), (17, 9, 777, 5, -1, 40, 64),
                                                      (1, 64, 300, 0, 1, 100, 64), (129, 129, 1000, 0, 1, 129, 64),
-                                                     (33, 16, 4096, 7, -1, 64, 32)])
+                                                     (33, 16, 4096, 7, -1, 64, 32), (64, 32, 300001, 9, -1, 8192, 64),
+                                                     (64, 32, 600003, 4, 1, 8192, 32)])
 def test_coef_gemm(L, pkg, P, K, Lc, p0, sgn, cmod, oprec):
+    """out[p] = 0.7 sum_q coef(q, col_p) X[q] against numpy; the last two
+    cases have outputs larger than the last-level cache (~300 MB)."""
     print(f"coef_gemm P={P} K={K} L={Lc} p0={p0} sgn={sgn} cmod={cmod} prec={oprec}", flush=True)
     rng = np.random.default_rng(P + K)
     coef = crandn(rng, K, cmod)          # coef[q][c], cp = 1, cq = cmod
