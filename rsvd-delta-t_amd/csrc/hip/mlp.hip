@@ -40,58 +40,145 @@ __global__ void k_mlp_features(int64_t n, const double *__restrict__ q, double x
 // ----------------------------------------------------------------- BF16 GEMM
 // C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) =
 // B[k*sbk + n*sbn], FP32 in memory, BF16 into the MFMA, FP32 accumulation.
-// A wave owns a 16 x 16 tile; a workgroup 4 tiles stacked in m.
+// A workgroup (4 waves) owns a 64 x 64 tile of C and a wave a 32 x 32
+// quarter (2 x 2 MFMA tiles).  Per K-step of 32 the workgroup loads the A and
+// B tiles once, rounded to BF16, into LDS with k contiguous -- from either
+// memory order of each operand, the loads of a wave running along the
+// operand's unit-stride dimension -- so each element of A is read once per
+// 64 columns of C and each element of B once per 64 rows.
 // MFMA 16x16x32 lane layout (checked on gfx950 with a known-answer product):
 //   A(16x32): lane l -> A[l%16][8(l/16)+i];  B(32x16): lane l -> B[8(l/16)+i][l%16];
 //   D(16x16): lane l -> D[4(l/16)+j][l%16].
 // Epilogues: 0 C = acc; 1 z = acc + bias, Z = z, C = silu(z); 2 C = acc + bias;
-// 3 C = acc * silu'(Z) (the backward of epilogue 1).
+// 3 C = acc * silu'(Z) (the backward of epilogue 1).  Long reductions (the
+// weight gradients, K = the batch) are split over gridDim.z: each split
+// writes its partial tile to `part` and k_gemm_reduce sums the splits in a
+// fixed order and applies the epilogue.
 enum { EPI_NONE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3 };
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A, int64_t sam,
-                                                   int64_t sak, const float *__restrict__ B, int64_t sbk,
-                                                   int64_t sbn, float *__restrict__ C, int64_t ldc,
-                                                   const float *__restrict__ bias, float *__restrict__ Z,
-                                                   int64_t ldz)
+__device__ __forceinline__ void gemm_epilogue(int64_t m, int64_t n, float v, float *__restrict__ C, int64_t ldc,
+                                              const float *__restrict__ bias, float *__restrict__ Z, int64_t ldz)
 {
+    if (EPI == EPI_BIAS_SILU) {
+        const float z = v + bias[n];
+        Z[m * ldz + n] = z;
+        v = z * sigm(z);
+    } else if (EPI == EPI_BIAS) {
+        v += bias[n];
+    } else if (EPI == EPI_DSILU) {
+        const float z = Z[m * ldz + n], s = sigm(z);
+        v *= s * (1.0f + z * (1.0f - s));
+    }
+    C[m * ldc + n] = v;
+}
+
+constexpr int GT = 64;          // tile of C per workgroup
+constexpr int GK = 32;          // K-step
+constexpr int GP = GK + 8;      // LDS row pitch in BF16 (80 B: 16-byte reads of 16 rows spread over the banks)
+
+// One 64 x 32 operand tile, X(r, k) = X[r*sr + k*sk] for rows r0.. (< R) and
+// k in [k0, ke): element e = 256 i + tid of the tile (i < 8) runs along the
+// operand's unit-stride dimension, so each load instruction of a wave reads
+// contiguous memory.  gemm_fetch loads into registers (FP32), gemm_put
+// stores them to LDS as [row][k] BF16.
+template <bool KF>
+__device__ __forceinline__ void gemm_fetch(float (&v)[8], const float *__restrict__ X, int64_t sr, int64_t sk,
+                                           int64_t r0, int64_t R, int64_t k0, int64_t ke)
+{
+    const int tid = threadIdx.x;
+    if constexpr (KF) {                // k contiguous: rows 8i + tid/32, k = tid % 32
+        const int64_t row = r0 + (tid >> 5), k = k0 + (tid & 31);
+        const float *p = X + row * sr + k;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (row + 8 * i < R && k < ke) ? p[8 * i * sr] : 0.0f;
+    } else {                           // rows contiguous: row tid % 64, k = 4i + tid/64
+        const int64_t row = r0 + (tid & 63), k = k0 + (tid >> 6);
+        const float *p = X + row * sr + k * sk;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (row < R && k + 4 * i < ke) ? p[4 * i * sk] : 0.0f;
+    }
+}
+template <bool KF>
+__device__ __forceinline__ void gemm_put(__bf16 *__restrict__ S, const float (&v)[8])
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int rr = KF ? 8 * i + (tid >> 5) : tid & 63, kk = KF ? tid & 31 : 4 * i + (tid >> 6);
+        S[rr * GP + kk] = (__bf16)v[i];
+    }
+}
+
+template <int EPI, bool AK, bool BK>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchunk, const float *__restrict__ A,
+                                                   int64_t sam, int64_t sak, const float *__restrict__ B, int64_t sbk,
+                                                   int64_t sbn, float *__restrict__ C, int64_t ldc,
+                                                   const float *__restrict__ bias, float *__restrict__ Z, int64_t ldz,
+                                                   float *__restrict__ part)
+{
+    __shared__ __align__(16) __bf16 As[GT * GP];
+    __shared__ __align__(16) __bf16 Bs[GT * GP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t m0 = ((int64_t)blockIdx.y * 4 + wv) * 16, n0 = (int64_t)blockIdx.x * 16;
-    if (m0 >= M) return;                       // no barrier in this kernel
+    const int64_t m0 = (int64_t)blockIdx.y * GT, n0 = (int64_t)blockIdx.x * GT;
+    const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+    const int64_t kb = (int64_t)blockIdx.z * kchunk;
+    const int64_t ke = kb + kchunk < K ? kb + kchunk : K;
     const int r = lane & 15, g = lane >> 4;
-    const int64_t am = m0 + r, bn = n0 + r;
-    f4v acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a, b;
+    f4v acc[2][2];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t k = k0 + 8 * g + i;
-            const float av = (am < M && k < K) ? A[am * sam + k * sak] : 0.0f;
-            const float bv = (bn < N && k < K) ? B[k * sbk + bn * sbn] : 0.0f;
-            a[i] = (__bf16)av;
-            b[i] = (__bf16)bv;
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[t][u] = (f4v){0, 0, 0, 0};
+    // the next K-step's operands are fetched into registers while this one's
+    // MFMAs run
+    float va[8], vb[8];
+    gemm_fetch<AK>(va, A, sam, sak, m0, M, kb, ke);
+    gemm_fetch<BK>(vb, B, sbn, sbk, n0, N, kb, ke);
+    for (int64_t k0 = kb; k0 < ke; k0 += GK) {
+        gemm_put<AK>(As, va);
+        gemm_put<BK>(Bs, vb);
+        __syncthreads();
+        if (k0 + GK < ke) {
+            gemm_fetch<AK>(va, A, sam, sak, m0, M, k0 + GK, ke);
+            gemm_fetch<BK>(vb, B, sbn, sbk, n0, N, k0 + GK, ke);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) a[t] = *(const bf16x8 *)(As + (wm + 16 * t + r) * GP + 8 * g);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8 *)(Bs + (wn + 16 * u + r) * GP + 8 * g);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int64_t m = m0 + 4 * g + j, n = n0 + r;
-        if (m >= M || n >= N) continue;
-        float v = acc[j];
-        if (EPI == EPI_BIAS_SILU) {
-            const float z = v + bias[n];
-            Z[m * ldz + n] = z;
-            v = z * sigm(z);
-        } else if (EPI == EPI_BIAS) {
-            v += bias[n];
-        } else if (EPI == EPI_DSILU) {
-            const float z = Z[m * ldz + n], s = sigm(z);
-            v *= s * (1.0f + z * (1.0f - s));
-        }
-        C[m * ldc + n] = v;
-    }
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t m = m0 + wm + 16 * t + 4 * g + j, n = n0 + wn + 16 * u + r;
+                if (m >= M || n >= N) continue;
+                if (part != nullptr) part[((int64_t)blockIdx.z * M + m) * N + n] = acc[t][u][j];
+                else gemm_epilogue<EPI>(m, n, acc[t][u][j], C, ldc, bias, Z, ldz);
+            }
+}
+
+// C = epilogue(sum over the nsplit partial products, in split order)
+template <int EPI>
+__global__ void k_gemm_reduce(int64_t M, int64_t N, int nsplit, const float *__restrict__ part, float *__restrict__ C,
+                              int64_t ldc, const float *__restrict__ bias, float *__restrict__ Z, int64_t ldz)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M * N) return;
+    float v = 0.0f;
+    for (int sp = 0; sp < nsplit; ++sp) v += part[sp * M * N + e];
+    gemm_epilogue<EPI>(e / N, e % N, v, C, ldc, bias, Z, ldz);
 }
 
 // out[n] = sum_m A[m*lda + n] (bias gradients); out must be zeroed first.
@@ -194,6 +281,11 @@ extern "C" int rsvdt_hip_mlp_features(int64_t n, const double *q, double xs, dou
     return RSVDT_HIP_OK;
 }
 
+// Split-K workspace: grown on demand, kept for the process (GEMMs of one
+// stream use it one after another).
+static float *g_gemm_part = nullptr;
+static size_t g_gemm_part_bytes = 0;
+
 extern "C" int rsvdt_hip_gemm_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam, int64_t sak,
                                    const float *B, int64_t sbk, int64_t sbn, float *C, int64_t ldc, int epilogue,
                                    const float *bias, float *Z, int64_t ldz, rsvdt_stream_t s)
@@ -204,14 +296,61 @@ extern "C" int rsvdt_hip_gemm_bf16(int64_t M, int64_t N, int64_t K, const float 
         set_error_msg("rsvdt_hip_gemm_bf16: bad arguments");
         return RSVDT_HIP_ERR_ARG;
     }
-    dim3 grid(ceil_div(N, 16), ceil_div(M, 64));
-    hipStream_t st = as_stream(s);
-    switch (epilogue) {
-    case 0: hipLaunchKernelGGL(k_gemm_bf16<0>, grid, dim3(256), 0, st, (int)M, (int)N, (int)K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Z, ldz); break;
-    case 1: hipLaunchKernelGGL(k_gemm_bf16<1>, grid, dim3(256), 0, st, (int)M, (int)N, (int)K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Z, ldz); break;
-    case 2: hipLaunchKernelGGL(k_gemm_bf16<2>, grid, dim3(256), 0, st, (int)M, (int)N, (int)K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Z, ldz); break;
-    default: hipLaunchKernelGGL(k_gemm_bf16<3>, grid, dim3(256), 0, st, (int)M, (int)N, (int)K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Z, ldz); break;
+    if (ceil_div(M, GT) > 65535) {
+        set_error_msg("rsvdt_hip_gemm_bf16: M above 64 x 65535");
+        return RSVDT_HIP_ERR_ARG;
     }
+    const int64_t tiles = ceil_div(M, GT) * ceil_div(N, GT);
+    // split the reduction when the tiles alone leave most CUs idle and K is long
+    int64_t nsplit = 1;
+    if (tiles < 256 && K >= 2048) {
+        nsplit = 512 / tiles;
+        const int64_t kmax = K / 512;                 // at least 512 per split
+        if (nsplit > kmax) nsplit = kmax;
+        if (nsplit > 64) nsplit = 64;
+        if (nsplit < 1) nsplit = 1;
+    }
+    int64_t kchunk = K;
+    if (nsplit > 1) {
+        kchunk = ceil_div(ceil_div(K, nsplit), GK) * GK;
+        nsplit = ceil_div(K, kchunk);
+    }
+    if (kchunk < 1) kchunk = 1;
+    float *part = nullptr;
+    if (nsplit > 1) {
+        const size_t need = (size_t)nsplit * (size_t)M * (size_t)N * sizeof(float);
+        if (need > g_gemm_part_bytes) {
+            if (g_gemm_part) rsvdt_hip_free(g_gemm_part);
+            g_gemm_part = nullptr;
+            g_gemm_part_bytes = 0;
+            if (rsvdt_hip_malloc((void **)&g_gemm_part, need)) return RSVDT_HIP_ERR_RUNTIME;
+            g_gemm_part_bytes = need;
+        }
+        part = g_gemm_part;
+    }
+    dim3 grid((unsigned)ceil_div(N, GT), (unsigned)ceil_div(M, GT), (unsigned)nsplit);
+    hipStream_t st = as_stream(s);
+    // AK: A k-contiguous (sak == 1), else read along its rows; BK: likewise for B
+    const bool ak = sak == 1, bk = sbk == 1;
+#define RSVDT_GEMM_K(E, AKV, BKV)                                                                                \
+    hipLaunchKernelGGL((k_gemm_bf16<E, AKV, BKV>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, (int)kchunk, A, \
+                       sam, sak, B, sbk, sbn, C, ldc, bias, Z, ldz, part)
+#define RSVDT_GEMM_LAUNCH(E)                                                                                     \
+    if (ak && bk) RSVDT_GEMM_K(E, true, true);                                                                  \
+    else if (ak) RSVDT_GEMM_K(E, true, false);                                                                  \
+    else if (bk) RSVDT_GEMM_K(E, false, true);                                                                  \
+    else RSVDT_GEMM_K(E, false, false);                                                                         \
+    if (part)                                                                                                    \
+        hipLaunchKernelGGL(k_gemm_reduce<E>, dim3((unsigned)ceil_div(M * N, 256)), dim3(256), 0, st, M, N,       \
+                           (int)nsplit, part, C, ldc, bias, Z, ldz)
+    switch (epilogue) {
+    case 0: RSVDT_GEMM_LAUNCH(0); break;
+    case 1: RSVDT_GEMM_LAUNCH(1); break;
+    case 2: RSVDT_GEMM_LAUNCH(2); break;
+    default: RSVDT_GEMM_LAUNCH(3); break;
+    }
+#undef RSVDT_GEMM_LAUNCH
+#undef RSVDT_GEMM_K
     RSVDT_LAUNCH_CHECK("rsvdt_hip_gemm_bf16");
     return RSVDT_HIP_OK;
 }
