@@ -452,7 +452,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=200)
     ap.add_argument("--eval", type=int, default=50)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=1000)
     ap.add_argument("--target-q", type=int, default=3,
                     help="power iterations of the plain runs whose forcing modes are the training targets")
     ap.add_argument("--seed", type=int, default=2024)
