@@ -271,6 +271,13 @@ PetscErrorCode RSVDT_BenchSteps(RSVD_matrices *, RSVDt_vars *, LNS_vars *, DFT_m
  * test vectors are taken from this host matrix (N x ncols_learned*Nw_eff,
  * frequency-major, column-major complex) and the remaining ones are random. */
 PetscErrorCode RSVDT_SetSketchColumns(PetscInt N, PetscInt ncols_learned, PetscInt Nw_eff, const double *host);
+/* The same from device memory in the sketch network's output layout, FP32
+ * complex interleaved, point-major: column j of frequency block m at row r is
+ * dev[(m N + r) ld + 2j] + i dev[(m N + r) ld + 2j + 1].  The array must stay
+ * valid until CreateRandomMat has run; no host copy is made.  Setting either
+ * form (or RSVDT_SetSketchColumns(0, 0, 0, NULL)) replaces the other. */
+PetscErrorCode RSVDT_SetSketchColumnsDevice(PetscInt N, PetscInt ncols_learned, PetscInt Nw_eff, const float *dev,
+                                            PetscInt ld);
 
 #ifdef __cplusplus
 }

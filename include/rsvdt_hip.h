@@ -47,6 +47,11 @@ int rsvdt_hip_malloc(void **p, size_t bytes);
 int rsvdt_hip_free(void *p);
 int rsvdt_hip_release_cached(void);
 size_t rsvdt_hip_cached_bytes(void);
+/* Y block m (N x k row-major complex FP64), columns j < ncols, from src (FP32
+ * complex interleaved, point-major): Y[(m N + r) k + j] = src[(m N + r) ld + 2j]
+ * + i src[(m N + r) ld + 2j + 1] (learned sketch columns, no counterpart) */
+int rsvdt_hip_sketch_cols_f32(int64_t nwe, int64_t N, int64_t ncols, int64_t k, const float *src, int64_t ld,
+                              double *Y, rsvdt_stream_t s);
 /* synchronous; both wait for all queued device work first */
 int rsvdt_hip_memcpy_h2d(void *dst, const void *src_host, size_t bytes);
 int rsvdt_hip_memcpy_d2h(void *dst_host, const void *src, size_t bytes);

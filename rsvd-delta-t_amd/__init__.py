@@ -111,12 +111,13 @@ ENTRY_POINTS = [
     "RSVDT_MatCreateDense", "RSVDT_MatCreateDenseBlocked", "RSVDT_MatGetDense", "RSVDT_MatCreateAIJ",
     "RSVDT_MatGetSize", "RSVDT_MatGetAIJ", "RSVDT_MatDestroy", "RSVDT_MatLoad", "RSVDT_MatView", "RSVDT_GetOptions",
     "RSVDT_SetOptions", "RSVDT_GetActionStats", "RSVDT_BenchSteps", "RSVDT_SetSketchColumns",
+    "RSVDT_SetSketchColumnsDevice",
     "TransientRunRK4", "TransientRemovalEst", "TSTransRK4", "RSVDT_VecCreate", "RSVDT_VecGetValues",
     "RSVDT_VecGetSize", "RSVDT_VecDestroy", "RSVDT_VecLoad", "RSVDT_VecView",
 ]
 HIP_ENTRY_POINTS = [
     "rsvdt_hip_init", "rsvdt_hip_last_error", "rsvdt_hip_malloc", "rsvdt_hip_free", "rsvdt_hip_release_cached",
-    "rsvdt_hip_cached_bytes", "rsvdt_hip_memcpy_h2d",
+    "rsvdt_hip_cached_bytes", "rsvdt_hip_sketch_cols_f32", "rsvdt_hip_memcpy_h2d",
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event", "rsvdt_hip_stream_sync",
     "rsvdt_hip_host_alloc", "rsvdt_hip_host_free", "rsvdt_hip_memcpy_d2h_async",
@@ -166,6 +167,7 @@ def lib():
     L.RSVDT_SetOptions.argtypes = [P(RSVDT_Options)]
     L.RSVDT_GetActionStats.argtypes = [P(RSVDT_ActionStats)]
     L.RSVDT_SetSketchColumns.argtypes = [_i64, _i64, _i64, _vp]
+    L.RSVDT_SetSketchColumnsDevice.argtypes = [_i64, _i64, _i64, _vp, _i64]
     act = [P(RSVD_matrices), P(RSVDt_vars), P(LNS_vars), P(DFT_matrices), P(Weight_matrices), P(Directories),
            P(TS_removal_matrices)]
     for n in ("DirectActionRK4", "AdjointActionRK4", "PowerIterationRK4"):

@@ -132,6 +132,32 @@ extern "C" size_t rsvdt_hip_cached_bytes(void) {
     std::lock_guard<std::mutex> lk(g_alloc_mu);
     return g_cache_bytes;
 }
+// Learned sketch columns from the network's FP32 output (interleaved complex,
+// point-major, ld floats per point) into the first ncols columns of the
+// sketch's frequency blocks (N x k row-major complex FP64 each).
+__global__ void k_sketch_cols_f32(int64_t total, int ncols, int64_t k, const float *__restrict__ src, int64_t ld,
+                                  double2 *__restrict__ Y)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;      // point * ncols + column
+    if (e >= total) return;
+    const int64_t pt = e / ncols;
+    const int j = (int)(e - pt * ncols);
+    Y[pt * k + j] = make_double2((double)src[pt * ld + 2 * j], (double)src[pt * ld + 2 * j + 1]);
+}
+extern "C" int rsvdt_hip_sketch_cols_f32(int64_t nwe, int64_t N, int64_t ncols, int64_t k, const float *src,
+                                         int64_t ld, double *Y, rsvdt_stream_t s)
+{
+    if (nwe <= 0 || N <= 0 || ncols <= 0) return RSVDT_HIP_OK;
+    if (!src || !Y || ncols > k || ld < 2 * ncols || ncols > 0x7fffffff) {
+        set_error_msg("rsvdt_hip_sketch_cols_f32: bad arguments");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    const int64_t total = nwe * N * ncols;
+    hipLaunchKernelGGL(k_sketch_cols_f32, dim3(ceil_div(total, 256)), dim3(256), 0, as_stream(s), total, (int)ncols,
+                       k, src, ld, (double2 *)Y);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_sketch_cols_f32");
+    return RSVDT_HIP_OK;
+}
 // Host<->device copies first wait for the whole device: the engine's work runs
 // on non-blocking streams, which the null-stream hipMemcpy does not order with.
 extern "C" int rsvdt_hip_memcpy_h2d(void *dst, const void *src, size_t bytes) {

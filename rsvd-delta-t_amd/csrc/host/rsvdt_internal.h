@@ -93,7 +93,8 @@ typedef struct {
     RSVDT_ActionStats stats;
     /* learned-sketch columns for CreateRandomMat */
     double *sketch_host;
-    PetscInt sketch_N, sketch_cols, sketch_Nw_eff;
+    const float *sketch_dev;     /* device columns (RSVDT_SetSketchColumnsDevice), ld floats per point */
+    PetscInt sketch_N, sketch_cols, sketch_Nw_eff, sketch_ld;
     char inputs_file[PETSC_MAX_PATH_LEN];
 } rsvdt_runtime;
 

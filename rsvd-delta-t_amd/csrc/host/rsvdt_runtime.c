@@ -289,6 +289,7 @@ PetscErrorCode RSVDT_SetSketchColumns(PetscInt N, PetscInt ncols, PetscInt Nw_ef
 {
     free(g_rsvdt.sketch_host);
     g_rsvdt.sketch_host = NULL;
+    g_rsvdt.sketch_dev = NULL;
     g_rsvdt.sketch_cols = 0;
     if (!host || ncols <= 0) return 0;
     size_t n = (size_t)N * (size_t)ncols * (size_t)Nw_eff * 2;
@@ -296,6 +297,17 @@ PetscErrorCode RSVDT_SetSketchColumns(PetscInt N, PetscInt ncols, PetscInt Nw_ef
     if (!g_rsvdt.sketch_host) return rsvdt_err(RSVDT_ERR_MEM, "RSVDT_SetSketchColumns", "out of memory");
     memcpy(g_rsvdt.sketch_host, host, n * sizeof(double));
     g_rsvdt.sketch_N = N; g_rsvdt.sketch_cols = ncols; g_rsvdt.sketch_Nw_eff = Nw_eff;
+    return 0;
+}
+
+PetscErrorCode RSVDT_SetSketchColumnsDevice(PetscInt N, PetscInt ncols, PetscInt Nw_eff, const float *dev, PetscInt ld)
+{
+    RSVDT_CHK(RSVDT_SetSketchColumns(0, 0, 0, NULL));
+    if (!dev || ncols <= 0) return 0;
+    if (N <= 0 || Nw_eff <= 0 || ld < 2 * ncols)
+        return rsvdt_err(RSVDT_ERR_USER, "RSVDT_SetSketchColumnsDevice", "need N, Nw_eff > 0 and ld >= 2 ncols");
+    g_rsvdt.sketch_dev = dev;
+    g_rsvdt.sketch_N = N; g_rsvdt.sketch_cols = ncols; g_rsvdt.sketch_Nw_eff = Nw_eff; g_rsvdt.sketch_ld = ld;
     return 0;
 }
 

@@ -417,9 +417,14 @@ PetscErrorCode CreateRandomMat(RSVD_matrices *R, RSVDt_vars *RSVDt, Directories 
     RSVDT_CHK(rsvdt_mat_alloc_dense(Nb, k * Nwe, k, 0, &Y));
     RSVDT_HIPCHK("CreateRandomMat", rsvdt_hip_fill_random(Nb, k, Nwe, seed, gauss, Y->d, g_rsvdt.stream));
     PetscInt kl = 0;
-    if (g_rsvdt.sketch_host && g_rsvdt.sketch_N == Nb && g_rsvdt.sketch_Nw_eff == Nwe) kl = g_rsvdt.sketch_cols;
+    if ((g_rsvdt.sketch_host || g_rsvdt.sketch_dev) && g_rsvdt.sketch_N == Nb && g_rsvdt.sketch_Nw_eff == Nwe)
+        kl = g_rsvdt.sketch_cols;
     if (kl > k) kl = k;
-    if (kl > 0) {
+    if (kl > 0 && g_rsvdt.sketch_dev) {
+        /* device columns (FP32 interleaved, point-major): converted in place into Y's blocks */
+        RSVDT_HIPCHK("CreateRandomMat", rsvdt_hip_sketch_cols_f32(Nwe, Nb, kl, k, g_rsvdt.sketch_dev, g_rsvdt.sketch_ld,
+                                                                  Y->d, g_rsvdt.stream));
+    } else if (kl > 0) {
         /* learned columns: host N x (kl*Nwe) column-major, column m*kl + j -> block m, column j */
         size_t n = (size_t)Nb * (size_t)kl * (size_t)Nwe;
         double *tmp = malloc(sizeof(double) * 2 * n);

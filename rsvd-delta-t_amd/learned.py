@@ -96,6 +96,7 @@ def _bind(L):
     L.rsvdt_hip_sketch_loss.argtypes = [I64, I64, I64, V, V, V, V, D, V]
     L.rsvdt_hip_adam.argtypes = [I64, V, V, V, V, F, F, F, F, F, F, V]
     L.RSVDT_SetSketchColumns.argtypes = [I64, I64, I64, V]
+    L.RSVDT_SetSketchColumnsDevice.argtypes = [I64, I64, I64, V, I64]
 
 
 def family_operator(mu0, Ur, N=500):
@@ -256,8 +257,7 @@ class SketchNet:
         return hist
 
     # -------------------------------------------------------------- inference
-    def columns(self, mu0, Ur, x, om):
-        """Learned columns (len(om) x len(x) x NCOL complex) for one family member."""
+    def _infer(self, mu0, Ur, x, om):
         n = len(x) * len(om)
         if n > self.B:
             self._alloc(n)
@@ -267,9 +267,21 @@ class SketchNet:
         q[:, :, 2] = mu0
         q[:, :, 3] = Ur
         self.q.put(q.reshape(n, 4))
-        out = self.forward_dev(n, self.q.p).get(np.float32, (n, 16)).astype(np.float64)
-        c = out[:, 0::2] + 1j * out[:, 1::2]
+        return n, self.forward_dev(n, self.q.p)
+
+    def columns(self, mu0, Ur, x, om):
+        """Learned columns (len(om) x len(x) x NCOL complex) for one family member."""
+        n, out = self._infer(mu0, Ur, x, om)
+        o = out.get(np.float32, (n, 16)).astype(np.float64)
+        c = o[:, 0::2] + 1j * o[:, 1::2]
         return c.reshape(len(om), len(x), NCOL)
+
+    def columns_dev(self, mu0, Ur, x, om):
+        """The same left on the device in the network's output layout (FP32
+        interleaved, point-major): (pointer, ld, Nw_eff, N) for
+        RSVDT_SetSketchColumnsDevice; valid until the next forward pass."""
+        _, out = self._infer(mu0, Ur, x, om)
+        return out.p, 16, len(om), len(x)
 
 
 # ------------------------------------------------------------------ family runs
@@ -279,9 +291,10 @@ def _session(pkg, A, k, q, **opts):
     return pkg.Session(A, k=k, q=q, Nw=Nw, w=w, TL=FAMILY["TL"], dt=T / 4096, transient_removal=True, **opts)
 
 
-def rsvd_run(pkg, A, k, q, seed, learned=None, **opts):
+def rsvd_run(pkg, A, k, q, seed, learned=None, learned_dev=None, **opts):
     """Plain RSVD-Delta-t (main.c sequence) on A; learned: optional host array
-    (Nw_eff x N x ncol complex) of sketch columns; opts: engine options
+    (Nw_eff x N x ncol complex) of sketch columns, or learned_dev: the same on
+    the device as SketchNet.columns_dev returns it; opts: engine options
     (precision, rk4_form, fuse_small, ...).  Returns (S_hat (k x Nw_eff),
     V_hat (N x k*Nw_eff), wall seconds)."""
     L = pkg.lib()
@@ -292,10 +305,13 @@ def rsvd_run(pkg, A, k, q, seed, learned=None, **opts):
         nwe, N, kl = learned.shape
         host = np.asfortranarray(learned.transpose(1, 0, 2).reshape(N, nwe * kl))
         pkg.check(L.RSVDT_SetSketchColumns(N, kl, nwe, host.ctypes.data_as(ctypes.c_void_p)), "RSVDT_SetSketchColumns")
+    elif learned_dev is not None:
+        ptr, ld, nwe, N = learned_dev
+        pkg.check(L.RSVDT_SetSketchColumnsDevice(N, NCOL, nwe, ptr, ld), "RSVDT_SetSketchColumnsDevice")
     try:
         s.random_sketch(seed=seed, gaussian=True)
     finally:
-        if learned is not None:
+        if learned is not None or learned_dev is not None:
             L.RSVDT_SetSketchColumns(0, 0, 0, None)
     s.direct()
     s.power_iteration()
@@ -424,8 +440,8 @@ def time_to_modes(pkg, net, members, k=16, q_ref=4, tol=1e-6, qmax=4, seed=4242,
         for name, opts, lrn in (("plain", plain, False), ("fast", fast, True)) + tuple(extra):
             for q in range(qmax + 1):
                 t0 = time.time()
-                cols = net.columns(mu0, Ur, x, om) if lrn else None
-                S, _, _ = rsvd_run(pkg, A, k, q, seed + i, learned=cols, **opts)
+                cols = net.columns_dev(mu0, Ur, x, om) if lrn else None
+                S, _, _ = rsvd_run(pkg, A, k, q, seed + i, learned_dev=cols, **opts)
                 wall = time.time() - t0
                 err = sigma1_error(S, Sref)
                 if err <= tol or q == qmax:

@@ -33,3 +33,40 @@ def test_learned_sketch_gain(pkg):
     # the learned columns only replace random ones: every run still ends in valid gains
     for r in rows:
         assert all(np.isfinite(r["learned"][q][0]) for q in (0, 1))
+
+
+def test_device_columns_match_host(pkg):
+    """RSVDT_SetSketchColumnsDevice (the network's FP32 output read in place)
+    gives the sketch the same entries as RSVDT_SetSketchColumns with the
+    columns taken through the host: the learned columns in the first NCOL
+    columns of every frequency block, the rest random from the same seed."""
+    import ctypes
+    L = pkg.lib()
+    learned._bind(L)
+    net = learned.SketchNet(seed=3)
+    N, Nw, w = 120, 16, 0.05
+    A = learned.family_operator(0.25, 2.0, N)
+    x = np.linspace(-85, 85, N)
+    om = learned.omegas(Nw, w, False)
+    cols = net.columns(0.25, 2.0, x, om)                     # Nw_eff x N x NCOL, through the host
+    print(f"device sketch columns: N={N} Nw={Nw} ncol={learned.NCOL}", flush=True)
+    Ys = []
+    for dev in (False, True):
+        s = pkg.Session(A, k=16, q=0, Nw=Nw, w=w, TL=1.0, dt=2 * np.pi / w / 256, transient_removal=False)
+        if dev:
+            ptr, ld, nwe, n = net.columns_dev(0.25, 2.0, x, om)
+            assert L.RSVDT_SetSketchColumnsDevice(n, learned.NCOL, nwe, ptr, ld) == 0
+        else:
+            host = np.asfortranarray(cols.transpose(1, 0, 2).reshape(N, len(om) * learned.NCOL))
+            assert L.RSVDT_SetSketchColumns(N, learned.NCOL, len(om), host.ctypes.data_as(ctypes.c_void_p)) == 0
+        try:
+            s.random_sketch(seed=5, gaussian=True)
+        finally:
+            L.RSVDT_SetSketchColumns(0, 0, 0, None)
+        Ys.append(s.get_Y())
+        s.close()
+    assert np.array_equal(Ys[0], Ys[1])
+    Y = Ys[1].reshape(N, len(om), 16)                          # column m*16 + j: block m, column j
+    for m in range(len(om)):
+        assert np.array_equal(Y[:, m, :learned.NCOL], cols[m])
+    assert np.all(np.abs(Y[:, :, learned.NCOL:]) > 0)
