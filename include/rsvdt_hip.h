@@ -52,6 +52,9 @@ size_t rsvdt_hip_cached_bytes(void);
  * + i src[(m N + r) ld + 2j + 1] (learned sketch columns, no counterpart) */
 int rsvdt_hip_sketch_cols_f32(int64_t nwe, int64_t N, int64_t ncols, int64_t k, const float *src, int64_t ld,
                               double *Y, rsvdt_stream_t s);
+/* iDFT table of CreateDFTiDFTMats (SourceCode/CreateDFTiDFTMats.c:42-57,93)
+ * built on the device: out[j nwe + m] = conj(exp(-2 pi i m_alias j / nt2)) 2/nt2 */
+int rsvdt_hip_idft_table(int64_t nwe, int64_t nt2, int real_op, double *out, rsvdt_stream_t s);
 /* synchronous; both wait for all queued device work first */
 int rsvdt_hip_memcpy_h2d(void *dst, const void *src_host, size_t bytes);
 int rsvdt_hip_memcpy_d2h(void *dst_host, const void *src, size_t bytes);

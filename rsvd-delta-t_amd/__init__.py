@@ -117,7 +117,7 @@ ENTRY_POINTS = [
 ]
 HIP_ENTRY_POINTS = [
     "rsvdt_hip_init", "rsvdt_hip_last_error", "rsvdt_hip_malloc", "rsvdt_hip_free", "rsvdt_hip_release_cached",
-    "rsvdt_hip_cached_bytes", "rsvdt_hip_sketch_cols_f32", "rsvdt_hip_memcpy_h2d",
+    "rsvdt_hip_cached_bytes", "rsvdt_hip_sketch_cols_f32", "rsvdt_hip_idft_table", "rsvdt_hip_memcpy_h2d",
     "rsvdt_hip_memcpy_d2h", "rsvdt_hip_memcpy_d2d", "rsvdt_hip_zero", "rsvdt_hip_sync", "rsvdt_hip_mem_info",
     "rsvdt_hip_stream_create", "rsvdt_hip_stream_destroy", "rsvdt_hip_stream_wait_event", "rsvdt_hip_stream_sync",
     "rsvdt_hip_host_alloc", "rsvdt_hip_host_free", "rsvdt_hip_memcpy_d2h_async",

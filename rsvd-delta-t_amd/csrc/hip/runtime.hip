@@ -132,6 +132,36 @@ extern "C" size_t rsvdt_hip_cached_bytes(void) {
     std::lock_guard<std::mutex> lk(g_alloc_mu);
     return g_cache_bytes;
 }
+// The iDFT table of CreateDFTiDFTMats: entry (m, j), m < nwe, j < nt2, at
+// out[j nwe + m] = conj(exp(alpha m_alias j)) 2/nt2, alpha = -2 pi i/nt2,
+// m_alias = m for a real operator, else m below nwe/2 (nwe even; (nwe-1)/2
+// inclusive when odd) and m - nwe + nt2 above.  The phase index
+// (m_alias j) mod nt2 is exact in integers, so the angle never exceeds 2 pi.
+__global__ void k_idft_table(int64_t nwe, int64_t nt2, int real_op, double2 *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nwe * nt2) return;
+    const int64_t j = e / nwe, m = e - j * nwe;
+    int64_t a = m;
+    if (!real_op && m > (nwe % 2 == 0 ? nwe / 2 - 1 : (nwe - 1) / 2)) a = m - nwe + nt2;
+    a %= nt2;
+    const int64_t p = (a * j) % nt2;
+    const double ph = (-2.0 * 3.14159265358979323846 / (double)nt2) * (double)p;
+    const double sc = 2.0 / (double)nt2;
+    out[e] = make_double2(cos(ph) * sc, -sin(ph) * sc);
+}
+extern "C" int rsvdt_hip_idft_table(int64_t nwe, int64_t nt2, int real_op, double *out, rsvdt_stream_t s)
+{
+    if (nwe <= 0 || nt2 <= 0 || !out || nt2 > 0x7fffffff) {
+        set_error_msg("rsvdt_hip_idft_table: bad arguments");
+        return RSVDT_HIP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_idft_table, dim3(ceil_div(nwe * nt2, 256)), dim3(256), 0, as_stream(s), nwe, nt2, real_op,
+                       (double2 *)out);
+    RSVDT_LAUNCH_CHECK("rsvdt_hip_idft_table");
+    return RSVDT_HIP_OK;
+}
+
 // Learned sketch columns from the network's FP32 output (interleaved complex,
 // point-major, ld floats per point) into the first ncols columns of the
 // sketch's frequency blocks (N x k row-major complex FP64 each).

@@ -255,43 +255,15 @@ PetscErrorCode CreateDFTiDFTMats(RSVDt_vars *RSVDt, DFT_matrices *D, LNS_vars *L
     (void)LNS;
     PetscInt Nt2 = 2 * RSVDt->TS.Ns, Nwe = RSVDt->RSVD.Nw_eff, Nw = RSVDt->RSVD.Nw;
     if (Nwe <= 0 || Nt2 <= 0) return rsvdt_err(RSVDT_ERR_USER, "CreateDFTiDFTMats", "grid not set up");
-    double *idft = malloc(sizeof(double) * 2 * (size_t)(Nwe * Nt2));
     double *dft = malloc(sizeof(double) * 2 * (size_t)(Nw * Nwe));
-    double *tw = malloc(sizeof(double) * 2 * (size_t)Nt2);
-    PetscInt *mal = malloc(sizeof(PetscInt) * 2 * (size_t)Nwe), *pos = mal + Nwe;
-    if (!idft || !dft || !tw || !mal) {
-        free(idft); free(dft); free(tw); free(mal);
-        return rsvdt_err(RSVDT_ERR_MEM, "CreateDFTiDFTMats", "out of memory");
-    }
-    /* exp(alpha*m_alias*j) depends only on (m_alias*j) mod 2Ns: one table of
-     * the 2Ns roots of unity (conjugated and scaled by 2/(2Ns)), walked with an
-     * integer phase per frequency, in place of a cos and a sin per entry.  The
-     * argument reduction is exact, so the entries equal the direct form's to
-     * rounding; the idft columns are written contiguously. */
-    const double ai = -2.0 * PI_D / (double)Nt2;             /* alpha = i*ai */
-    for (PetscInt m = 0; m < Nt2; m++) {
-        const double ph = ai * (double)m;
-        tw[2 * m] = cos(ph) * (2.0 / (double)Nt2);
-        tw[2 * m + 1] = -sin(ph) * (2.0 / (double)Nt2);       /* conj */
-    }
-    for (PetscInt iw = 0; iw < Nwe; iw++) {
-        PetscInt a;
-        if (RSVDt->TS.RealOperator) a = iw;
-        else if (Nwe % 2 == 0) a = (iw <= Nwe / 2 - 1) ? iw : (iw - Nwe) + Nt2;
-        else a = (iw <= (Nwe - 1) / 2) ? iw : (iw - Nwe) + Nt2;
-        mal[iw] = ((a % Nt2) + Nt2) % Nt2;
-        pos[iw] = 0;
-    }
-    for (PetscInt j = 0; j < Nt2; j++) {
-        double *col = idft + 2 * (size_t)j * (size_t)Nwe;
-        for (PetscInt iw = 0; iw < Nwe; iw++) {
-            col[2 * iw] = tw[2 * pos[iw]];
-            col[2 * iw + 1] = tw[2 * pos[iw] + 1];
-            pos[iw] += mal[iw];
-            if (pos[iw] >= Nt2) pos[iw] -= Nt2;
-        }
-    }
-    free(tw); free(mal);
+    if (!dft) return rsvdt_err(RSVDT_ERR_MEM, "CreateDFTiDFTMats", "out of memory");
+    /* idft: built on the device from the exact integer phase (m_alias j) mod
+     * 2Ns of each entry (k_idft_table), Nw_eff x 2Ns column-major == block
+     * width 1; ordered before every reader on the engine's stream */
+    PetscErrorCode ierr = rsvdt_mat_alloc_dense(Nwe, Nt2, 1, 0, &D->idft);
+    if (ierr) { free(dft); return ierr; }
+    int hr = rsvdt_hip_idft_table(Nwe, Nt2, RSVDt->TS.RealOperator ? 1 : 0, D->idft->d, g_rsvdt.stream);
+    if (hr) { free(dft); RSVDT_MatDestroy(&D->idft); return rsvdt_err(hr, "CreateDFTiDFTMats", "%s", rsvdt_hip_last_error()); }
     const double a2 = -2.0 * PI_D / (RSVDt->TS.RealOperator ? (double)(2 * Nwe) : (double)Nwe);
     for (PetscInt iw = 0; iw < Nwe; iw++)
         for (PetscInt s = 0; s < Nw; s++) {
@@ -299,11 +271,9 @@ PetscErrorCode CreateDFTiDFTMats(RSVDt_vars *RSVDt, DFT_matrices *D, LNS_vars *L
             dft[2 * ((size_t)iw * Nw + s)] = cos(ph);
             dft[2 * ((size_t)iw * Nw + s) + 1] = sin(ph);
         }
-    /* idft: Nw_eff x 2Ns column-major == block width 1 */
-    PetscErrorCode ierr = RSVDT_MatCreateDenseBlocked(Nwe, Nt2, 1, idft, &D->idft);
     /* dft: Nw x Nw_eff, stored row-major (one block of Nw_eff columns) */
-    if (!ierr) ierr = RSVDT_MatCreateDenseBlocked(Nw, Nwe, Nwe, dft, &D->dft);
-    free(idft); free(dft);
+    ierr = RSVDT_MatCreateDenseBlocked(Nw, Nwe, Nwe, dft, &D->dft);
+    free(dft);
     return ierr;
 }
 

@@ -209,3 +209,32 @@ def test_graph_replay(pkg, tr, direct, prec):
         assert np.array_equal(a, b)
     if prec == 64:
         assert rel(out[1][0], o.action(F, direct=direct)) < 1e-10
+
+
+@pytest.mark.parametrize("real,Nw,Ns", [(False, 16, 512), (True, 64, 4096), (False, 15, 300)])
+def test_idft_table_on_device(pkg, real, Nw, Ns):
+    """The iDFT table CreateDFTiDFTMats builds on the device against the
+    closed form idft(m, j) = exp(+2 pi i m_alias j / (2 Ns)) 2/(2 Ns)
+    (SourceCode/CreateDFTiDFTMats.c:42-57,93), m_alias the aliased frequency
+    index of a complex operator (odd Nw included)."""
+    import ctypes
+    print(f"idft table real={real} Nw={Nw} Ns={Ns}", flush=True)
+    L = pkg.lib()
+    V = pkg.RSVDt_vars()
+    Nwe = Nw // 2 if real else Nw
+    V.RSVD.Nw, V.RSVD.Nw_eff, V.TS.Ns, V.TS.RealOperator = Nw, Nwe, Ns, int(real)
+    D, LNS = pkg.DFT_matrices(), pkg.LNS_vars()
+    pkg.check(L.RSVDT_Initialize(None, None), "RSVDT_Initialize")
+    pkg.check(L.CreateDFTiDFTMats(ctypes.byref(V), ctypes.byref(D), ctypes.byref(LNS)), "CreateDFTiDFTMats")
+    idft = pkg.mat_to_dense(pkg._vp(D.idft))                        # Nw_eff x 2Ns
+    pkg.destroy(D.idft)
+    pkg.destroy(D.dft)
+    nt2 = 2 * Ns
+    m = np.arange(Nwe)
+    if not real:
+        half = Nwe // 2 - 1 if Nwe % 2 == 0 else (Nwe - 1) // 2
+        m = np.where(m > half, m - Nwe + nt2, m)
+    p = np.outer(m.astype(np.int64), np.arange(nt2, dtype=np.int64)) % nt2     # exact phase index
+    ref = np.exp(2j * np.pi * p / nt2) * (2.0 / nt2)
+    assert idft.shape == (Nwe, nt2)
+    assert np.max(np.abs(idft - ref)) < 1e-15 * (2.0 / nt2) * 8
