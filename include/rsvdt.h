@@ -274,7 +274,9 @@ PetscErrorCode RSVDT_SetSketchColumns(PetscInt N, PetscInt ncols_learned, PetscI
 /* The same from device memory in the sketch network's output layout, FP32
  * complex interleaved, point-major: column j of frequency block m at row r is
  * dev[(m N + r) ld + 2j] + i dev[(m N + r) ld + 2j + 1].  The array must stay
- * valid until CreateRandomMat has run; no host copy is made.  Setting either
+ * valid until CreateRandomMat has run; no host copy is made, and
+ * CreateRandomMat waits for the whole device before reading it (the columns
+ * may come from work on any stream).  Setting either
  * form (or RSVDT_SetSketchColumns(0, 0, 0, NULL)) replaces the other. */
 PetscErrorCode RSVDT_SetSketchColumnsDevice(PetscInt N, PetscInt ncols_learned, PetscInt Nw_eff, const float *dev,
                                             PetscInt ld);

@@ -391,7 +391,10 @@ PetscErrorCode CreateRandomMat(RSVD_matrices *R, RSVDt_vars *RSVDt, Directories 
         kl = g_rsvdt.sketch_cols;
     if (kl > k) kl = k;
     if (kl > 0 && g_rsvdt.sketch_dev) {
-        /* device columns (FP32 interleaved, point-major): converted in place into Y's blocks */
+        /* device columns (FP32 interleaved, point-major): converted in place into
+         * Y's blocks, after the whole device is idle -- the caller may have
+         * produced them on any stream, which the engine's stream does not wait for */
+        RSVDT_HIPCHK("CreateRandomMat", rsvdt_hip_sync());
         RSVDT_HIPCHK("CreateRandomMat", rsvdt_hip_sketch_cols_f32(Nwe, Nb, kl, k, g_rsvdt.sketch_dev, g_rsvdt.sketch_ld,
                                                                   Y->d, g_rsvdt.stream));
     } else if (kl > 0) {
