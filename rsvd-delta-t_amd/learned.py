@@ -375,13 +375,13 @@ def evaluate(pkg, net, members, qs=(0, 1, 2), k=16, q_ref=4, seed=9001, tol=1e-6
         nwe = Sref.shape[1]
         om = omegas(FAMILY["Nw"], FAMILY["w"], False)[:nwe]
         t0 = time.time()
-        cols = net.columns(mu0, Ur, x, om)
+        cols = net.columns_dev(mu0, Ur, x, om)      # read in place by each learned run below
         t_inf = time.time() - t0
         row = {"mu0": mu0, "Ur": Ur, "plain": {}, "learned": {}, "inference_s": t_inf}
         for q in qs:
             S, _, wall = rsvd_run(pkg, A, k, q, seed + i)
             row["plain"][q] = (sigma1_error(S, Sref), wall)
-            S, _, wall = rsvd_run(pkg, A, k, q, seed + i, learned=cols)
+            S, _, wall = rsvd_run(pkg, A, k, q, seed + i, learned_dev=cols)
             row["learned"][q] = (sigma1_error(S, Sref), wall + t_inf)
         rows.append(row)
         if log:
