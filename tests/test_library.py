@@ -207,3 +207,18 @@ def test_matview_reproduces_reference_file(pkg, tmp_path):
     nbytes = 32 + 8 * M + 8 * nz + 16 * nz
     got = open(out, "rb").read()
     assert len(got) == nbytes and got == ref[:nbytes]
+
+
+def test_device_sketch_columns_arguments(pkg, capfd):
+    """RSVDT_SetSketchColumnsDevice checks its sizes on the host (the device
+    array is not read until CreateRandomMat): ld below 2 ncols or an empty
+    grid is an error with the entry point's name; a null array clears any
+    columns set before."""
+    L = pkg.lib()
+    fake = ctypes.c_void_p(0x1000)          # never dereferenced by the checks
+    assert L.RSVDT_SetSketchColumnsDevice(500, 8, 32, fake, 15) != 0
+    assert "RSVDT_SetSketchColumnsDevice" in capfd.readouterr().err
+    assert L.RSVDT_SetSketchColumnsDevice(0, 8, 32, fake, 16) != 0
+    capfd.readouterr()
+    assert L.RSVDT_SetSketchColumnsDevice(500, 8, 32, None, 16) == 0
+    assert L.RSVDT_SetSketchColumns(0, 0, 0, None) == 0
