@@ -331,7 +331,10 @@ int rsvdt_hip_mlp_features(int64_t n, const double *q, double xs, double ws, dou
 /* C(m,n) = sum_k A(m,k) B(k,n) with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
  * BF16 operands (rounded to nearest even), FP32 accumulation, C row stride ldc.
  * epilogue 0: C = acc; 1: z = acc + bias[n], Z(m,n) = z, C = z sigmoid(z);
- * 2: C = acc + bias[n]; 3: C = acc * silu'(Z(m,n)).  Z row stride ldz. */
+ * 2: C = acc + bias[n]; 3: C = acc * silu'(Z(m,n)).  Z row stride ldz.
+ * Long reductions (K >= 2048 with few output tiles) are split and summed in a
+ * fixed order through one process-wide workspace: such calls must not
+ * overlap on different streams. */
 int rsvdt_hip_gemm_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam, int64_t sak, const float *B,
                         int64_t sbk, int64_t sbn, float *C, int64_t ldc, int epilogue, const float *bias, float *Z,
                         int64_t ldz, rsvdt_stream_t s);
